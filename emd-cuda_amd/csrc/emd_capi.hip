@@ -1,0 +1,443 @@
+// emd_capi.hip -- C ABI of libemd_gfx950.so (declared in include/emd_gfx950.h).
+#include "../../include/emd_gfx950.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "emd_sift.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(EMD_ERR_RUNTIME, "%s: %s", what, hipGetErrorString(e));
+}
+
+constexpr int kMaxPlainBlocks = 32768;  // blocks per plain sub-launch (bounds spill memory)
+
+}  // namespace
+
+struct emd_handle {
+  emd_config cfg;
+  int device = 0;
+  int num_cus = 0;
+  int lds_max = 0;
+  char* spill = nullptr;       // global knot tiles of the global-tile retry instances
+  size_t spill_size = 0;
+  int* defer_list = nullptr;   // traces deferred by the main launch
+  size_t defer_cap = 0;
+  unsigned* counters = nullptr;  // [0] queue head, [1] deferred count, [2] retry queue,
+                                 // [3] retry deferrals (always 0: the retry tile holds any
+                                 // trace of the launch)
+};
+
+namespace {
+
+int ensure_spill(emd_handle* h, size_t bytes) {
+  if (bytes <= h->spill_size) return EMD_OK;
+  if (h->spill) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize before growing scratch");
+    e = hipFree(h->spill);
+    h->spill = nullptr;
+    h->spill_size = 0;
+    if (e != hipSuccess) return hip_fail(e, "hipFree(scratch)");
+  }
+  size_t sz = std::max(bytes, (size_t)1 << 20);
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&h->spill), sz);
+  if (e != hipSuccess) {
+    h->spill = nullptr;
+    return fail(EMD_ERR_MEMORY, "hipMalloc(%zu bytes of spill scratch): %s", sz,
+                hipGetErrorString(e));
+  }
+  h->spill_size = sz;
+  return EMD_OK;
+}
+
+int ensure_defer(emd_handle* h, int B) {
+  if ((size_t)B <= h->defer_cap) return EMD_OK;
+  if (h->defer_list) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize before growing the retry list");
+    (void)hipFree(h->defer_list);
+    h->defer_list = nullptr;
+    h->defer_cap = 0;
+  }
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&h->defer_list), (size_t)B * sizeof(int));
+  if (e != hipSuccess) {
+    h->defer_list = nullptr;
+    return fail(EMD_ERR_MEMORY, "hipMalloc(retry list of %d traces): %s", B,
+                hipGetErrorString(e));
+  }
+  h->defer_cap = (size_t)B;
+  return EMD_OK;
+}
+
+// knot slots that can never be exceeded for traces of length <= n
+int cap_slots(const emd_config& c, int n) {
+  return 2 * std::min(c.max_extrema, std::max(n, 1)) + 4 * c.nbsym;
+}
+
+struct Shape {
+  int wg, chunk, tile, global_tile;
+};
+
+int chunk_of(int wg, int max_n) { return emdk::chunk_for(max_n, wg); }
+
+int shape_chunk(int& wg, int max_n) {
+  if (wg != 64 && wg != 128 && wg != 256)
+    return fail(EMD_ERR_VALUE, "threads per workgroup must be 64, 128 or 256, got %d", wg);
+  int chunk = emdk::chunk_for(max_n, wg);
+  while (chunk > 64 && wg < 256) {  // a longer trace needs more threads per workgroup
+    wg *= 2;
+    chunk = emdk::chunk_for(max_n, wg);
+  }
+  if (chunk > 64)
+    return fail(EMD_ERR_VALUE, "trace length %d exceeds the supported maximum %d", max_n,
+                emdk::kMaxN);
+  return EMD_OK;
+}
+
+int lds_fit_slots(const emd_handle* h, int wg) {
+  return (h->lds_max - emdk::static_lds_bytes(wg)) / emdk::kSlotBytes;
+}
+
+// Main launch: tile_req slots (0 = the capacity rule), clamped to what LDS holds.
+int make_shape(const emd_handle* h, int wg, int tile_req, int max_n, Shape* s) {
+  int rc = shape_chunk(wg, max_n);
+  if (rc) return rc;
+  const int cap = cap_slots(h->cfg, max_n);
+  int tile = tile_req > 0 ? std::min(tile_req, cap) : cap;
+  tile = std::max(0, std::min(tile, lds_fit_slots(h, wg)));
+  s->wg = wg;
+  s->chunk = chunk_of(wg, max_n);
+  s->tile = tile;
+  s->global_tile = 0;
+  return EMD_OK;
+}
+
+// Retry launch: a tile that holds every trace of the launch (LDS if it fits, else global).
+int make_retry_shape(const emd_handle* h, int wg, int max_n, Shape* s) {
+  int rc = shape_chunk(wg, max_n);
+  if (rc) return rc;
+  const int cap = cap_slots(h->cfg, max_n);
+  if (cap <= lds_fit_slots(h, wg)) {
+    s->wg = wg;
+    s->chunk = chunk_of(wg, max_n);
+    s->tile = cap;
+    s->global_tile = 0;
+    return EMD_OK;
+  }
+  s->wg = 256;  // global-tile instances exist for 256-thread workgroups
+  s->chunk = chunk_of(256, max_n);
+  s->tile = cap;
+  s->global_tile = 1;
+  return EMD_OK;
+}
+
+emdk::SiftParams params(const emd_handle* h, const Shape& s) {
+  emdk::SiftParams P;
+  P.max_extrema = h->cfg.max_extrema;
+  P.nbsym = h->cfg.nbsym;
+  P.max_iteration = h->cfg.max_iteration;
+  P.max_imfs = h->cfg.max_imfs;
+  P.tile_slots = s.tile;
+  P.zero_fill = 1;
+  return P;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, what);
+  return EMD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* emd_last_error(void) { return g_last_error.c_str(); }
+
+int emd_version(void) { return 1; }
+
+int emd_static_lds_bytes(int threads_per_block) {
+  return emdk::static_lds_bytes(threads_per_block);
+}
+
+int emd_lds_slot_bytes(void) { return emdk::kSlotBytes; }
+
+int emd_device_info(int* lds_bytes_per_workgroup, int* num_cus, int* warp_size) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  hipDeviceProp_t p;
+  e = hipGetDeviceProperties(&p, dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+  if (lds_bytes_per_workgroup) *lds_bytes_per_workgroup = (int)p.sharedMemPerBlock;
+  if (num_cus) *num_cus = p.multiProcessorCount;
+  if (warp_size) *warp_size = p.warpSize;
+  return EMD_OK;
+}
+
+int emd_create(const emd_config* cfg, emd_handle** out) {
+  if (!cfg || !out) return fail(EMD_ERR_VALUE, "emd_create: null argument");
+  *out = nullptr;
+  if (cfg->max_extrema < 1 || cfg->max_extrema > 32767)
+    return fail(EMD_ERR_VALUE, "max_extrema must be in [1, 32767], got %d", cfg->max_extrema);
+  if (cfg->nbsym < 1 || cfg->nbsym > emdk::kMaxNbsym)
+    return fail(EMD_ERR_VALUE, "nbsym must be in [1, %d], got %d", emdk::kMaxNbsym,
+                cfg->nbsym);
+  if (cfg->max_iteration < 1)
+    return fail(EMD_ERR_VALUE, "max_iteration must be >= 1, got %d", cfg->max_iteration);
+  if (cfg->max_imfs < 1) return fail(EMD_ERR_VALUE, "max_imfs must be >= 1, got %d", cfg->max_imfs);
+  const int tpb = cfg->threads_per_block;
+  if (tpb != 64 && tpb != 128 && tpb != 256)
+    return fail(EMD_ERR_VALUE, "threads_per_block must be 64, 128 or 256 on gfx950, got %d", tpb);
+  if (cfg->tile_slots < 0) return fail(EMD_ERR_VALUE, "tile_slots must be >= 0");
+  emd_handle* h = new emd_handle();
+  h->cfg = *cfg;
+  hipError_t e = hipGetDevice(&h->device);
+  if (e != hipSuccess) {
+    delete h;
+    return hip_fail(e, "hipGetDevice");
+  }
+  hipDeviceProp_t p;
+  e = hipGetDeviceProperties(&p, h->device);
+  if (e != hipSuccess) {
+    delete h;
+    return hip_fail(e, "hipGetDeviceProperties");
+  }
+  if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0) {
+    std::string arch = p.gcnArchName;
+    delete h;
+    return fail(EMD_ERR_RUNTIME, "libemd_gfx950 needs a gfx950 (MI355X) device, found %s",
+                arch.c_str());
+  }
+  h->num_cus = p.multiProcessorCount;
+  h->lds_max = (int)p.sharedMemPerBlock;
+  e = hipMalloc(reinterpret_cast<void**>(&h->counters), 256);
+  if (e != hipSuccess) {
+    delete h;
+    return fail(EMD_ERR_MEMORY, "hipMalloc(counters): %s", hipGetErrorString(e));
+  }
+  *out = h;
+  return EMD_OK;
+}
+
+int emd_destroy(emd_handle* h) {
+  if (!h) return EMD_OK;
+  hipError_t e = hipDeviceSynchronize();
+  if (h->spill) (void)hipFree(h->spill);
+  if (h->defer_list) (void)hipFree(h->defer_list);
+  if (h->counters) (void)hipFree(h->counters);
+  delete h;
+  if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize in emd_destroy");
+  return EMD_OK;
+}
+
+static int launch_shape(emd_handle* h, const Shape& s, int persistent, int per_cu_req, int B,
+                        emdk::Batch b, hipStream_t st, const char* what) {
+  const emdk::SiftParams P = params(h, s);
+  int rc;
+  if (s.global_tile) {
+    rc = ensure_spill(h, (size_t)emdk::tile_bytes(s.tile) *
+                             (size_t)(persistent ? h->num_cus * 4 : std::min(B, kMaxPlainBlocks)));
+    if (rc) return rc;
+    b.spill = h->spill;
+    b.spill_stride = emdk::tile_bytes(s.tile);
+  }
+  if (persistent) {
+    const size_t lds = s.global_tile ? 0 : (size_t)s.tile * emdk::kSlotBytes;
+    const int fit = emdk::max_resident_wgs(1, s.global_tile, s.wg, s.chunk, lds);
+    if (fit <= 0) return fail(EMD_ERR_RUNTIME, "%s: kernel shape does not fit a CU", what);
+    int per_cu = per_cu_req;
+    if (per_cu <= 0 || per_cu > fit) per_cu = fit;
+    if (s.global_tile) per_cu = std::min(per_cu, 4);
+    int grid = h->num_cus * per_cu;
+    if (!b.count_ptr) grid = std::min(grid, B);
+    b.B = B;
+    b.base = 0;
+    rc = emdk::launch_sift(1, s.global_tile, s.wg, s.chunk, grid, b, P, st);
+    if (rc) return hip_fail((hipError_t)rc, what);
+    return check_launch(what);
+  }
+  for (int base = 0; base < B; base += kMaxPlainBlocks) {
+    const int nb = std::min(kMaxPlainBlocks, B - base);
+    b.B = nb;
+    b.base = base;
+    rc = emdk::launch_sift(0, s.global_tile, s.wg, s.chunk, nb, b, P, st);
+    if (rc) return hip_fail((hipError_t)rc, what);
+    rc = check_launch(what);
+    if (rc) return rc;
+  }
+  return EMD_OK;
+}
+
+__global__ void zero_counters_kernel(unsigned* c) {
+  if (threadIdx.x < 4) c[threadIdx.x] = 0u;
+}
+
+static int run_common(emd_handle* h, const double* values, const int64_t* offsets, int N,
+                      int B, int max_n, const int* order, const emd_launch& L, double* imfs,
+                      const int64_t* out_offsets, int* n_imfs, int* overflow, int* debug_n,
+                      int* debug_extno, int* debug_nzm, int* stats, hipStream_t st) {
+  if (B < 0) return fail(EMD_ERR_VALUE, "batch size must be >= 0, got %d", B);
+  if (max_n < 0) return fail(EMD_ERR_VALUE, "trace length must be >= 0, got %d", max_n);
+  if (B == 0) return EMD_OK;
+  if ((max_n > 0 && (!values || !imfs)) || !n_imfs || !overflow || !debug_n || !debug_extno ||
+      !debug_nzm)
+    return fail(EMD_ERR_VALUE, "null input or output pointer");
+  Shape s, rs;
+  int rc = make_shape(h, L.threads, L.tile_slots > 0 ? L.tile_slots : h->cfg.tile_slots,
+                      max_n, &s);
+  if (rc) return rc;
+  rc = make_retry_shape(h, s.wg, max_n, &rs);
+  if (rc) return rc;
+  rc = ensure_defer(h, B);
+  if (rc) return rc;
+  hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, st, h->counters);
+  rc = check_launch("counter reset");
+  if (rc) return rc;
+  emdk::Batch b;
+  b.values = values;
+  b.offsets = offsets;
+  b.N = N;
+  b.order = order;
+  b.imfs = imfs;
+  b.out_offsets = out_offsets;
+  b.n_imfs = n_imfs;
+  b.overflow = overflow;
+  b.dbg_n = debug_n;
+  b.dbg_e = debug_extno;
+  b.dbg_z = debug_nzm;
+  b.stats = stats;
+  b.spill = nullptr;
+  b.spill_stride = 0;
+  b.queue = h->counters;
+  b.count_ptr = nullptr;
+  b.defer_count = h->counters + 1;
+  b.defer_list = h->defer_list;
+  b.retry = 0;
+  rc = launch_shape(h, s, L.persistent, L.wgs_per_cu, B, b, st, "sift launch");
+  if (rc) return rc;
+  // Retry launch over the deferred traces (a device-side count; no host round trip).
+  emdk::Batch r = b;
+  r.order = h->defer_list;
+  r.queue = h->counters + 2;
+  r.count_ptr = h->counters + 1;
+  r.defer_count = h->counters + 3;
+  r.defer_list = h->defer_list;  // never written: the retry tile holds every trace
+  r.retry = 1;
+  return launch_shape(h, rs, 1, 0, B, r, st, "retry sift launch");
+}
+
+int emd_run_padded(emd_handle* h, const double* signals, int B, int N, double* imfs,
+                   int* n_imfs, int* overflow, int* debug_n, int* debug_extno,
+                   int* debug_nzm, int* stats, void* stream) {
+  if (!h) return fail(EMD_ERR_VALUE, "null handle");
+  if (N < 0 || N > emdk::kMaxN)
+    return fail(EMD_ERR_VALUE, "trace length N=%d outside [0, %d]", N, emdk::kMaxN);
+  emd_launch L;
+  L.threads = h->cfg.threads_per_block;
+  L.tile_slots = h->cfg.tile_slots;
+  L.wgs_per_cu = 0;
+  L.persistent = 0;
+  return run_common(h, signals, nullptr, N, B, N, nullptr, L, imfs, nullptr, n_imfs, overflow,
+                    debug_n, debug_extno, debug_nzm, stats, (hipStream_t)stream);
+}
+
+int emd_run_ragged(emd_handle* h, const double* values, const int64_t* offsets, int B,
+                   int max_n, const int* order, const emd_launch* launch, double* imfs,
+                   const int64_t* out_offsets, int* n_imfs, int* overflow, int* debug_n,
+                   int* debug_extno, int* debug_nzm, int* stats, void* stream) {
+  if (!h) return fail(EMD_ERR_VALUE, "null handle");
+  if (!offsets) return fail(EMD_ERR_VALUE, "ragged input needs CSR offsets");
+  if (max_n < 0 || max_n > emdk::kMaxN)
+    return fail(EMD_ERR_VALUE, "max trace length %d outside [0, %d]", max_n, emdk::kMaxN);
+  emd_launch L;
+  if (launch) {
+    L = *launch;
+  } else {
+    L.threads = h->cfg.threads_per_block;
+    L.tile_slots = h->cfg.tile_slots;
+    L.wgs_per_cu = 0;
+    L.persistent = 1;
+  }
+  return run_common(h, values, offsets, 0, B, max_n, order, L, imfs, out_offsets, n_imfs,
+                    overflow, debug_n, debug_extno, debug_nzm, stats, (hipStream_t)stream);
+}
+
+int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double* max_env,
+                      double* min_env, int* n_extrema, int* degenerate, double* max_kpos,
+                      double* max_kval, double* min_kpos, double* min_kval, int* n_mirrored,
+                      void* stream) {
+  if (!h) return fail(EMD_ERR_VALUE, "null handle");
+  if (N < 0 || N > emdk::kMaxN)
+    return fail(EMD_ERR_VALUE, "trace length N=%d outside [0, %d]", N, emdk::kMaxN);
+  if (B < 0) return fail(EMD_ERR_VALUE, "batch size must be >= 0, got %d", B);
+  if (B == 0) return EMD_OK;
+  Shape s;
+  int rc = make_retry_shape(h, 256, N, &s);  // a tile that holds every trace
+  if (rc) return rc;
+  const emdk::SiftParams P = params(h, s);
+  const int mbuf = h->cfg.max_extrema + 2 * h->cfg.nbsym;
+  if (s.global_tile) {
+    rc = ensure_spill(h, (size_t)emdk::tile_bytes(s.tile) * std::min(B, kMaxPlainBlocks));
+    if (rc) return rc;
+  }
+  for (int base = 0; base < B; base += kMaxPlainBlocks) {
+    const int nb = std::min(kMaxPlainBlocks, B - base);
+    emdk::EnvBatch e;
+    e.values = signals + (int64_t)base * N;
+    e.N = N;
+    e.B = nb;
+    e.max_env = max_env + (int64_t)base * N;
+    e.min_env = min_env + (int64_t)base * N;
+    e.n_extrema = n_extrema + 2 * (int64_t)base;
+    e.degenerate = degenerate + base;
+    e.max_kpos = max_kpos + (int64_t)base * mbuf;
+    e.max_kval = max_kval + (int64_t)base * mbuf;
+    e.min_kpos = min_kpos + (int64_t)base * mbuf;
+    e.min_kval = min_kval + (int64_t)base * mbuf;
+    e.n_mirrored = n_mirrored + 2 * (int64_t)base;
+    e.mbuf = mbuf;
+    e.spill = h->spill;
+    e.spill_stride = emdk::tile_bytes(s.tile);
+    rc = emdk::launch_envelope(s.global_tile, s.wg, s.chunk, e, P, (hipStream_t)stream);
+    if (rc) return hip_fail((hipError_t)rc, "envelope launch");
+    rc = check_launch("envelope kernel");
+    if (rc) return rc;
+  }
+  return EMD_OK;
+}
+
+int emd_max_resident(emd_handle* h, const emd_launch* launch, int max_n, int* wgs_per_cu) {
+  if (!h || !launch || !wgs_per_cu) return fail(EMD_ERR_VALUE, "null argument");
+  Shape s;
+  int rc = make_shape(h, launch->threads, launch->tile_slots > 0 ? launch->tile_slots
+                                                                 : h->cfg.tile_slots,
+                      max_n, &s);
+  if (rc) return rc;
+  *wgs_per_cu = emdk::max_resident_wgs(launch->persistent ? 1 : 0, 0, s.wg, s.chunk,
+                                       (size_t)s.tile * emdk::kSlotBytes);
+  return EMD_OK;
+}
+
+}  // extern "C"

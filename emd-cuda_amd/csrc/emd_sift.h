@@ -1,0 +1,85 @@
+// emd_sift.h -- declarations shared by the gfx950 sift kernels and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace emdk {
+
+constexpr int kMaxNbsym = 4;      // mirrored points per side supported by the kernels
+constexpr int kMaxN = 16384;      // longest trace: 256 threads x 64 samples in registers
+constexpr int kSlotBytes = 44;    // LDS bytes per knot slot: 5 fp64 + 1 int32
+
+struct SiftParams {
+  int max_extrema;    // per-side extrema cap (overflow rule); counts are clamped to it
+  int nbsym;          // mirrored points per side (1..kMaxNbsym)
+  int max_iteration;  // sift cap per IMF
+  int max_imfs;       // output rows per trace (IMF cap = max_imfs - 1, then the residue)
+  int tile_slots;     // knot slots held in LDS (both envelopes)
+  int zero_fill;      // write zeros into output rows at and beyond n_imfs
+};
+
+// Addressing of one sift launch (uniform [B,N] rows or CSR traces).
+struct Batch {
+  const double* values;
+  const int64_t* offsets;      // CSR offsets [B+1] or null (uniform rows of length N)
+  int N;                       // uniform length when offsets == null
+  int B;                       // traces handled by this launch
+  int base;                    // first queue position of this launch
+  const int* order;            // processing order or null (FIFO)
+  double* imfs;
+  const int64_t* out_offsets;  // per-signal output base (elements) or null (padded)
+  int* n_imfs;
+  int* overflow;
+  int* dbg_n;
+  int* dbg_e;
+  int* dbg_z;
+  int* stats;
+  char* spill;                 // per-workgroup global knot tiles (global-tile instances)
+  int64_t spill_stride;        // bytes per workgroup
+  unsigned* queue;             // persistent-kernel queue head
+  const unsigned* count_ptr;   // queue length in device memory (retry launches) or null
+  unsigned* defer_count;       // traces deferred by this launch (device counter)
+  int* defer_list;             // their indices
+  int retry;                   // 1 for a retry launch (sets stats flag bit 1)
+};
+
+// Envelope-only pass outputs (reference envelope_batch_kernel ABI, kernels.py:1100-1113).
+struct EnvBatch {
+  const double* values;
+  int N;
+  int B;
+  double* max_env;
+  double* min_env;
+  int* n_extrema;   // [B,2]
+  int* degenerate;  // [B]
+  double* max_kpos; // [B,mbuf]
+  double* max_kval;
+  double* min_kpos;
+  double* min_kval;
+  int* n_mirrored;  // [B,2]
+  int mbuf;
+  char* spill;
+  int64_t spill_stride;
+};
+
+
+// Bytes of one workgroup's global knot tile of `slots` slots.
+int64_t tile_bytes(int slots);
+// Static LDS of the sift kernels for a workgroup size.
+int static_lds_bytes(int wg);
+// Smallest supported register chunk (samples per thread) for a trace length.
+int chunk_for(int n, int wg);
+
+// Launch one sift kernel instance.  persistent != 0 selects the queue kernel, whose grid
+// is `grid` workgroups; otherwise grid is the number of traces of the launch.  global_tile
+// selects the instance whose knot tile lives in global memory (b.spill).  Returns a
+// hipError_t.
+int launch_sift(int persistent, int global_tile, int wg, int chunk, int grid, const Batch& b,
+                const SiftParams& P, hipStream_t stream);
+int launch_envelope(int global_tile, int wg, int chunk, const EnvBatch& e,
+                    const SiftParams& P, hipStream_t stream);
+int reset_queue(unsigned* q, hipStream_t stream);
+// Resident workgroups per CU for a kernel instance at the given dynamic LDS.
+int max_resident_wgs(int persistent, int global_tile, int wg, int chunk, size_t dyn_lds);
+
+}  // namespace emdk

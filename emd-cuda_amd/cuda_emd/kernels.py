@@ -1,0 +1,277 @@
+"""kernels.py -- EMDKernelConfig and EMDBatchLauncher over the gfx950 C ABI.
+
+Same names, fields and call signatures as the reference's kernels.py
+(SeismicFlow/emd-cuda kernels.py:1170-1407); the device code is the AOT-compiled HIP
+library libemd_gfx950.so instead of an NVRTC-compiled CUDA string.  Arrays are torch
+tensors on the current ROCm device (numpy inputs are copied there).
+
+float64 only, exactly as the reference: EMDKernelConfig.dtype is a read-only property.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+# LDS per workgroup on gfx950 (MI355X); used when no device is queried.
+GFX950_LDS_BYTES = 163840
+
+
+def _torch():
+    import torch
+    return torch
+
+
+@dataclass
+class EMDKernelConfig:
+    """Configuration of the batched sift (reference EMDKernelConfig, kernels.py:1170-1176).
+
+    ``max_extrema`` is the per-side extrema cap: counts are clamped to it and a trace whose
+    count reaches it is flagged in ``overflow``.  ``tile_slots`` (addition) is the LDS knot
+    tile of the one-workgroup-per-trace launch; 0 sizes it from ``max_extrema`` like the
+    reference's shared-memory budget.  Traces that outgrow the tile are redone by a retry
+    launch with a tile that holds them, so the tile never changes results.
+    """
+
+    max_extrema: int = 1024
+    nbsym: int = 2
+    max_iteration: int = 1000
+    max_imfs: int = 16
+    threads_per_block: int = 128
+    tile_slots: int = 0
+
+    @property
+    def dtype(self) -> np.dtype:
+        return np.dtype(np.float64)
+
+    @property
+    def real_t_name(self) -> str:
+        return "double"
+
+    @property
+    def itemsize(self) -> int:
+        return self.dtype.itemsize
+
+    def static_shared_mem_bytes(self) -> int:
+        """Static LDS of a sift workgroup (edge exchange + reduction scratch)."""
+        return int(_lib.load().emd_static_lds_bytes(self.threads_per_block))
+
+    def shared_mem_bytes(self) -> int:
+        """Dynamic LDS of the knot tile: slots for both envelopes x bytes per slot
+        (5 fp64 + 1 int32).  Reference counterpart: kernels.py:1273-1275."""
+        slots = self.tile_slots or (2 * self.max_extrema + 4 * self.nbsym)
+        return slots * int(_lib.load().emd_lds_slot_bytes())
+
+    @classmethod
+    def for_device(cls, device=None, target_blocks_per_sm: int = 4,
+                   min_max_extrema: int = 1024, lds_bytes: int | None = None) -> "EMDKernelConfig":
+        """Size ``max_extrema`` for occupancy like the reference (kernels.py:1205-1271):
+        budget LDS for ``target_blocks_per_sm`` workgroups per CU, never below
+        ``min_max_extrema``, and never beyond 95% of the per-workgroup LDS."""
+        cfg = cls()
+        if lds_bytes is None:
+            lds = ctypes.c_int(0)
+            if device is not None:
+                torch = _torch()
+                with torch.cuda.device(device):
+                    _lib.check(_lib.load().emd_device_info(ctypes.byref(lds), None, None),
+                               "EMDKernelConfig.for_device")
+            else:
+                _lib.check(_lib.load().emd_device_info(ctypes.byref(lds), None, None),
+                           "EMDKernelConfig.for_device")
+            lds_bytes = lds.value
+        static_bytes = cfg.static_shared_mem_bytes()
+        per_extremum = 2 * int(_lib.load().emd_lds_slot_bytes())  # one slot per envelope
+        budget = max(0, lds_bytes // target_blocks_per_sm - static_bytes)
+        occ = max(0, budget // per_extremum - 2 * cfg.nbsym)
+        desired = max(min_max_extrema, occ)
+        hard = max(64, (int(lds_bytes * 0.95) - static_bytes) // per_extremum - 2 * cfg.nbsym)
+        cfg.max_extrema = min(desired, hard)
+        return cfg
+
+    def build(self) -> "EMDBatchLauncher":
+        return EMDBatchLauncher(None, self)
+
+
+def _as_device_f64(x, what: str):
+    """torch float64 CUDA tensor from a torch tensor or numpy array; dtype is checked,
+    never cast (reference: kernels.py:1346-1352)."""
+    torch = _torch()
+    if isinstance(x, torch.Tensor):
+        if x.dtype != torch.float64:
+            raise ValueError(
+                f"This launcher was compiled for REAL_T=double (dtype float64), but got "
+                f"{what}.dtype={x.dtype}. Passing mismatched dtypes would silently "
+                f"reinterpret bytes rather than convert them -- cast explicitly first.")
+        if not x.is_cuda:
+            x = x.cuda()
+        return x.contiguous()
+    a = np.asarray(x)
+    if a.dtype != np.float64:
+        raise ValueError(
+            f"This launcher was compiled for REAL_T=double (dtype float64), but got "
+            f"{what}.dtype={a.dtype}. Passing mismatched dtypes would silently reinterpret "
+            f"bytes rather than convert them -- cast explicitly first.")
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _ptr(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr() if t is not None and t.numel() > 0 else 0)
+
+
+def _stream():
+    return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+class EMDBatchLauncher:
+    """Owns the library handle (reference EMDBatchLauncher, kernels.py:1289-1407).
+
+    ``source`` is accepted for signature compatibility and ignored: the kernels are
+    compiled ahead of time for gfx950.
+    """
+
+    def __init__(self, source=None, config: EMDKernelConfig | None = None):
+        if config is None and isinstance(source, EMDKernelConfig):
+            source, config = None, source
+        self.config = config or EMDKernelConfig()
+        lib = _lib.load()
+        c = self.config
+        cc = _lib.EmdConfigC(c.max_extrema, c.nbsym, c.max_iteration, c.max_imfs,
+                             c.threads_per_block, c.tile_slots)
+        h = ctypes.c_void_p()
+        _lib.check(lib.emd_create(ctypes.byref(cc), ctypes.byref(h)), "EMDBatchLauncher")
+        self._h = h
+        self.last_stats = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().emd_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # -- helpers ----------------------------------------------------------------------
+    def _check_T(self, T, N):
+        if T is None:
+            return
+        Tn = T.detach().cpu().numpy() if hasattr(T, "detach") else np.asarray(T)
+        if Tn.dtype != np.float64:
+            raise ValueError(
+                f"This launcher was compiled for REAL_T=double (dtype float64), but got "
+                f"T.dtype={Tn.dtype}.")
+        if Tn.shape != (N,):
+            raise ValueError(f"T must have shape ({N},), got {Tn.shape}")
+
+    def run(self, signals, T=None) -> tuple:
+        """Decompose ``signals`` [B,N] float64.  Returns (imfs [B,max_imfs,N], n_imfs [B],
+        overflow [B], debug_n, debug_extno, debug_nzm [B,max_imfs]) as torch tensors; the
+        per-trace statistics [B,4] (sifts, peak extrema, knot sum, flags) are kept in
+        ``last_stats``.  T is validated but, as in PyEMD's 'simple' extrema mode, the
+        sample index is the time axis."""
+        torch = _torch()
+        s = _as_device_f64(signals, "signals")
+        if s.dim() != 2:
+            raise ValueError(f"signals must be [batch, N], got shape {tuple(s.shape)}")
+        B, N = s.shape
+        self._check_T(T, N)
+        c = self.config
+        dev = s.device
+        imfs = torch.empty((B, c.max_imfs, N), dtype=torch.float64, device=dev)
+        n_imfs = torch.zeros((B,), dtype=torch.int32, device=dev)
+        overflow = torch.zeros((B,), dtype=torch.int32, device=dev)
+        dn = torch.full((B, c.max_imfs), -1, dtype=torch.int32, device=dev)
+        de = torch.full((B, c.max_imfs), -1, dtype=torch.int32, device=dev)
+        dz = torch.full((B, c.max_imfs), -1, dtype=torch.int32, device=dev)
+        stats = torch.zeros((B, 4), dtype=torch.int32, device=dev)
+        _lib.check(_lib.load().emd_run_padded(
+            self._h, _ptr(s), B, N, _ptr(imfs), _ptr(n_imfs), _ptr(overflow), _ptr(dn),
+            _ptr(de), _ptr(dz), _ptr(stats), _stream()), "EMDBatchLauncher.run")
+        self.last_stats = stats
+        return imfs, n_imfs, overflow, dn, de, dz
+
+    def run_ragged(self, values, offsets, order=None, threads=None, tile_slots=0,
+                   wgs_per_cu=0, persistent=True, out_offsets=None) -> tuple:
+        """Decompose CSR traces (values, int64 offsets [B+1]).  ``order`` (int32 queue
+        order) never changes outputs.  Returns (imfs flat, out_offsets [B] int64, n_imfs,
+        overflow, debug_n, debug_extno, debug_nzm, stats); row k of trace s is
+        imfs[out_offsets[s] + k*len(s) : ... + len(s)]."""
+        torch = _torch()
+        v = _as_device_f64(values, "values")
+        off = offsets if isinstance(offsets, torch.Tensor) else torch.from_numpy(
+            np.ascontiguousarray(np.asarray(offsets, dtype=np.int64)))
+        if off.dtype != torch.int64:
+            raise ValueError(f"offsets must be int64, got {off.dtype}")
+        off_host = off.cpu().numpy()
+        off = off.cuda().contiguous()
+        B = int(off.numel()) - 1
+        if B < 0:
+            raise ValueError("offsets needs at least one entry")
+        lengths = np.diff(off_host)
+        if B and (lengths.min() < 0 or off_host[-1] > v.numel()):
+            raise ValueError("offsets must be non-decreasing and within values")
+        max_n = int(lengths.max()) if B else 0
+        c = self.config
+        dev = v.device
+        if out_offsets is None:
+            out_offsets = off[:-1] * c.max_imfs
+            total = int(off_host[-1]) * c.max_imfs
+        else:
+            out_offsets = out_offsets.cuda().contiguous()
+            total = int((out_offsets.cpu().numpy() + c.max_imfs * lengths).max()) if B else 0
+        imfs = torch.empty((max(total, 1),), dtype=torch.float64, device=dev)
+        n_imfs = torch.zeros((B,), dtype=torch.int32, device=dev)
+        overflow = torch.zeros((B,), dtype=torch.int32, device=dev)
+        dn = torch.full((B, c.max_imfs), -1, dtype=torch.int32, device=dev)
+        de = torch.full((B, c.max_imfs), -1, dtype=torch.int32, device=dev)
+        dz = torch.full((B, c.max_imfs), -1, dtype=torch.int32, device=dev)
+        stats = torch.zeros((B, 4), dtype=torch.int32, device=dev)
+        if order is not None:
+            order = (order if isinstance(order, torch.Tensor) else torch.from_numpy(
+                np.ascontiguousarray(np.asarray(order, dtype=np.int32)))).to(dev).int().contiguous()
+            if order.numel() != B:
+                raise ValueError(f"order must have {B} entries, got {order.numel()}")
+        L = _lib.EmdLaunchC(threads or c.threads_per_block, tile_slots, wgs_per_cu,
+                            1 if persistent else 0)
+        _lib.check(_lib.load().emd_run_ragged(
+            self._h, _ptr(v), _ptr(off), B, max_n, _ptr(order), ctypes.byref(L), _ptr(imfs),
+            _ptr(out_offsets), _ptr(n_imfs), _ptr(overflow), _ptr(dn), _ptr(de), _ptr(dz),
+            _ptr(stats), _stream()), "EMDBatchLauncher.run_ragged")
+        self.last_stats = stats
+        return imfs, out_offsets, n_imfs, overflow, dn, de, dz, stats
+
+    def run_envelope_only(self, signals, T=None) -> tuple:
+        """Extrema + mirror + both envelope splines of the raw signals, no sifting
+        (reference kernels.py:1374-1407).  Returns (max_env, min_env, n_extrema [B,2],
+        degenerate [B], max_knot_pos, max_knot_val, min_knot_pos, min_knot_val [B,MBUF],
+        n_mirrored [B,2])."""
+        torch = _torch()
+        s = _as_device_f64(signals, "signals")
+        if s.dim() != 2:
+            raise ValueError(f"signals must be [batch, N], got shape {tuple(s.shape)}")
+        B, N = s.shape
+        self._check_T(T, N)
+        c = self.config
+        mbuf = c.max_extrema + 2 * c.nbsym
+        dev = s.device
+        z = lambda *shape, dt=torch.float64: torch.zeros(shape, dtype=dt, device=dev)
+        max_env, min_env = z(B, N), z(B, N)
+        n_extrema, degenerate = z(B, 2, dt=torch.int32), z(B, dt=torch.int32)
+        mkp, mkv, nkp, nkv = z(B, mbuf), z(B, mbuf), z(B, mbuf), z(B, mbuf)
+        n_mirrored = z(B, 2, dt=torch.int32)
+        _lib.check(_lib.load().emd_envelope_only(
+            self._h, _ptr(s), B, N, _ptr(max_env), _ptr(min_env), _ptr(n_extrema),
+            _ptr(degenerate), _ptr(mkp), _ptr(mkv), _ptr(nkp), _ptr(nkv), _ptr(n_mirrored),
+            _stream()), "EMDBatchLauncher.run_envelope_only")
+        return max_env, min_env, n_extrema, degenerate, mkp, mkv, nkp, nkv, n_mirrored
+
+    def max_resident(self, threads: int, tile_slots: int, max_n: int, persistent=True) -> int:
+        L = _lib.EmdLaunchC(threads, tile_slots, 0, 1 if persistent else 0)
+        out = ctypes.c_int(0)
+        _lib.check(_lib.load().emd_max_resident(self._h, ctypes.byref(L), max_n,
+                                                 ctypes.byref(out)), "max_resident")
+        return out.value

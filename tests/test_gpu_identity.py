@@ -1,0 +1,74 @@
+"""Outputs never depend on the launch: plain vs persistent, workgroup size, LDS tile
+(including traces forced through the retry launch) and queue order give bit-identical
+results."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _padded(x, **cfg):
+    import torch
+    from cuda_emd import EMDKernelConfig
+    L = EMDKernelConfig(**cfg).build()
+    out = L.run(torch.from_numpy(x).cuda())
+    return [t.cpu().numpy() for t in out] + [L.last_stats.cpu().numpy()]
+
+
+def _ragged(values, offsets, order=None, **kw):
+    import torch
+    from cuda_emd import EMDKernelConfig
+    L = EMDKernelConfig().build()
+    out = L.run_ragged(torch.from_numpy(values).cuda(), offsets, order=order, **kw)
+    torch.cuda.synchronize()
+    return [t.cpu().numpy() for t in out]
+
+
+def _chirps(B=40):
+    from cuda_emd import synth
+    return synth.chirp_mix(B, 1024, seed=7)
+
+
+@pytest.mark.parametrize("tpb", [64, 128, 256])
+def test_workgroup_size_invariance(tpb):
+    x = _chirps()
+    ref = _padded(x)
+    got = _padded(x, threads_per_block=tpb)
+    for a, b in zip(ref[:6], got[:6]):
+        assert np.array_equal(a, b)
+
+
+def test_forced_retry_identical():
+    x = _chirps()
+    ref = _padded(x)
+    got = _padded(x, tile_slots=40)  # every trace outgrows the tile -> retry launch
+    for a, b in zip(ref[:6], got[:6]):
+        assert np.array_equal(a, b)
+    assert np.all(got[6][:, 3] & 2)
+
+
+def test_persistent_ragged_matches_plain():
+    from cuda_emd import synth
+    values, offsets, kinds = synth.ragged_mix(B=48, seed=11, n_lo=100, n_hi=3000)
+    lengths = np.diff(offsets)
+    a = _ragged(values, offsets, persistent=True, threads=256)
+    b = _ragged(values, offsets, persistent=False, threads=128)
+    rng = np.random.default_rng(0)
+    order = rng.permutation(len(lengths)).astype(np.int32)
+    c = _ragged(values, offsets, order=order, persistent=True, threads=128, tile_slots=300)
+    for k in range(2, 7):
+        assert np.array_equal(a[k], b[k]) and np.array_equal(a[k], c[k])
+    for s in range(len(lengths)):
+        n = int(a[2][s])
+        L = int(lengths[s])
+        base = int(a[1][s])
+        for arr in (b, c):
+            assert np.array_equal(a[0][base:base + n * L], arr[0][base:base + n * L])
+    # each ragged trace equals its own padded single-trace decomposition
+    for s in range(0, len(lengths), 7):
+        L = int(lengths[s])
+        x = values[offsets[s]:offsets[s + 1]][None, :]
+        p = _padded(x)
+        n = int(a[2][s])
+        assert n == int(p[1][0])
+        assert np.array_equal(a[0][int(a[1][s]):int(a[1][s]) + n * L].reshape(n, L), p[0][0, :n])
