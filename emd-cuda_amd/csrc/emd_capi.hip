@@ -96,59 +96,50 @@ int cap_slots(const emd_config& c, int n) {
 }
 
 struct Shape {
-  int wg, chunk, tile, global_tile;
+  int wg, tile, global_tile, max_n;
 };
 
-int chunk_of(int wg, int max_n) { return emdk::chunk_for(max_n, wg); }
-
-int shape_chunk(int& wg, int max_n) {
+int check_wg(int wg) {
   if (wg != 64 && wg != 128 && wg != 256)
     return fail(EMD_ERR_VALUE, "threads per workgroup must be 64, 128 or 256, got %d", wg);
-  int chunk = emdk::chunk_for(max_n, wg);
-  while (chunk > 64 && wg < 256) {  // a longer trace needs more threads per workgroup
-    wg *= 2;
-    chunk = emdk::chunk_for(max_n, wg);
-  }
-  if (chunk > 64)
-    return fail(EMD_ERR_VALUE, "trace length %d exceeds the supported maximum %d", max_n,
-                emdk::kMaxN);
   return EMD_OK;
 }
 
-int lds_fit_slots(const emd_handle* h, int wg) {
-  return (h->lds_max - emdk::static_lds_bytes(wg)) / emdk::kSlotBytes;
+// knot slots that fit next to the work area of traces up to max_n samples
+int lds_fit_slots(const emd_handle* h, int wg, int max_n) {
+  const long avail = (long)h->lds_max - emdk::static_lds_bytes(wg) -
+                     (long)emdk::dyn_lds_bytes(1, 0, max_n);
+  return avail > 0 ? (int)(avail / emdk::kSlotBytes) : 0;
 }
 
 // Main launch: tile_req slots (0 = the capacity rule), clamped to what LDS holds.
 int make_shape(const emd_handle* h, int wg, int tile_req, int max_n, Shape* s) {
-  int rc = shape_chunk(wg, max_n);
+  int rc = check_wg(wg);
   if (rc) return rc;
+  if (max_n > emdk::kMaxN)
+    return fail(EMD_ERR_VALUE, "trace length %d exceeds the supported maximum %d", max_n,
+                emdk::kMaxN);
   const int cap = cap_slots(h->cfg, max_n);
   int tile = tile_req > 0 ? std::min(tile_req, cap) : cap;
-  tile = std::max(0, std::min(tile, lds_fit_slots(h, wg)));
+  tile = std::max(0, std::min(tile, lds_fit_slots(h, wg, max_n)));
   s->wg = wg;
-  s->chunk = chunk_of(wg, max_n);
   s->tile = tile;
   s->global_tile = 0;
+  s->max_n = max_n;
+  if (emdk::dyn_lds_bytes(1, 0, max_n) + emdk::static_lds_bytes(wg) > (size_t)h->lds_max)
+    return fail(EMD_ERR_VALUE, "trace length %d does not fit the LDS work area", max_n);
   return EMD_OK;
 }
 
 // Retry launch: a tile that holds every trace of the launch (LDS if it fits, else global).
 int make_retry_shape(const emd_handle* h, int wg, int max_n, Shape* s) {
-  int rc = shape_chunk(wg, max_n);
+  int rc = make_shape(h, wg, 0, max_n, s);
   if (rc) return rc;
   const int cap = cap_slots(h->cfg, max_n);
-  if (cap <= lds_fit_slots(h, wg)) {
-    s->wg = wg;
-    s->chunk = chunk_of(wg, max_n);
+  if (cap > s->tile) {
     s->tile = cap;
-    s->global_tile = 0;
-    return EMD_OK;
+    s->global_tile = 1;
   }
-  s->wg = 256;  // global-tile instances exist for 256-thread workgroups
-  s->chunk = chunk_of(256, max_n);
-  s->tile = cap;
-  s->global_tile = 1;
   return EMD_OK;
 }
 
@@ -159,6 +150,7 @@ emdk::SiftParams params(const emd_handle* h, const Shape& s) {
   P.max_iteration = h->cfg.max_iteration;
   P.max_imfs = h->cfg.max_imfs;
   P.tile_slots = s.tile;
+  P.max_n = std::max(s.max_n, 1);
   P.zero_fill = 1;
   return P;
 }
@@ -264,8 +256,8 @@ static int launch_shape(emd_handle* h, const Shape& s, int persistent, int per_c
     b.spill_stride = emdk::tile_bytes(s.tile);
   }
   if (persistent) {
-    const size_t lds = s.global_tile ? 0 : (size_t)s.tile * emdk::kSlotBytes;
-    const int fit = emdk::max_resident_wgs(1, s.global_tile, s.wg, s.chunk, lds);
+    const size_t lds = emdk::dyn_lds_bytes(s.global_tile, s.tile, P.max_n);
+    const int fit = emdk::max_resident_wgs(1, s.global_tile, s.wg, lds);
     if (fit <= 0) return fail(EMD_ERR_RUNTIME, "%s: kernel shape does not fit a CU", what);
     int per_cu = per_cu_req;
     if (per_cu <= 0 || per_cu > fit) per_cu = fit;
@@ -274,7 +266,7 @@ static int launch_shape(emd_handle* h, const Shape& s, int persistent, int per_c
     if (!b.count_ptr) grid = std::min(grid, B);
     b.B = B;
     b.base = 0;
-    rc = emdk::launch_sift(1, s.global_tile, s.wg, s.chunk, grid, b, P, st);
+    rc = emdk::launch_sift(1, s.global_tile, s.wg, grid, b, P, st);
     if (rc) return hip_fail((hipError_t)rc, what);
     return check_launch(what);
   }
@@ -282,7 +274,7 @@ static int launch_shape(emd_handle* h, const Shape& s, int persistent, int per_c
     const int nb = std::min(kMaxPlainBlocks, B - base);
     b.B = nb;
     b.base = base;
-    rc = emdk::launch_sift(0, s.global_tile, s.wg, s.chunk, nb, b, P, st);
+    rc = emdk::launch_sift(0, s.global_tile, s.wg, nb, b, P, st);
     if (rc) return hip_fail((hipError_t)rc, what);
     rc = check_launch(what);
     if (rc) return rc;
@@ -420,7 +412,7 @@ int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double
     e.mbuf = mbuf;
     e.spill = h->spill;
     e.spill_stride = emdk::tile_bytes(s.tile);
-    rc = emdk::launch_envelope(s.global_tile, s.wg, s.chunk, e, P, (hipStream_t)stream);
+    rc = emdk::launch_envelope(s.global_tile, s.wg, e, P, (hipStream_t)stream);
     if (rc) return hip_fail((hipError_t)rc, "envelope launch");
     rc = check_launch("envelope kernel");
     if (rc) return rc;
@@ -435,8 +427,8 @@ int emd_max_resident(emd_handle* h, const emd_launch* launch, int max_n, int* wg
                                                                  : h->cfg.tile_slots,
                       max_n, &s);
   if (rc) return rc;
-  *wgs_per_cu = emdk::max_resident_wgs(launch->persistent ? 1 : 0, 0, s.wg, s.chunk,
-                                       (size_t)s.tile * emdk::kSlotBytes);
+  *wgs_per_cu = emdk::max_resident_wgs(launch->persistent ? 1 : 0, 0, s.wg,
+                                       emdk::dyn_lds_bytes(0, s.tile, std::max(max_n, 1)));
   return EMD_OK;
 }
 

@@ -5,19 +5,9 @@ namespace emdk {
 
 __global__ void reset_counter_kernel(unsigned* q) { *q = 0u; }
 
-#define EMD_DECL_WG(WG)                        \
-  void* sift_instance_##WG##_1(int which);     \
-  void* sift_instance_##WG##_2(int which);     \
-  void* sift_instance_##WG##_4(int which);     \
-  void* sift_instance_##WG##_8(int which);     \
-  void* sift_instance_##WG##_16(int which);    \
-  void* sift_instance_##WG##_32(int which);    \
-  void* sift_instance_##WG##_64(int which);
-EMD_DECL_WG(64)
-EMD_DECL_WG(128)
-EMD_DECL_WG(256)
-#undef EMD_DECL_WG
-
+void* sift_instance_64(int which);
+void* sift_instance_128(int which);
+void* sift_instance_256(int which);
 
 int64_t tile_bytes(int slots) {
   return (((int64_t)slots * kSlotBytes) + 255) / 256 * 256;
@@ -32,52 +22,33 @@ int static_lds_bytes(int wg) {
   return 0;
 }
 
-int chunk_for(int n, int wg) {
-  int per = (n + wg - 1) / wg;
-  if (per < 1) per = 1;
-  int c = 1;
-  while (c < per) c <<= 1;
-  return c;
+size_t dyn_lds_bytes(int global_tile, int tile_slots, int max_n) {
+  const size_t nbcap = (size_t)(max_n + 63) / 64;
+  return nbcap * kWorkBlockBytes + (global_tile ? 0 : (size_t)tile_slots * kSlotBytes);
 }
 
-#define EMD_SWITCH_C(WG, which, chunk)                \
-  switch (chunk) {                                    \
-    case 1: return sift_instance_##WG##_1(which);     \
-    case 2: return sift_instance_##WG##_2(which);     \
-    case 4: return sift_instance_##WG##_4(which);     \
-    case 8: return sift_instance_##WG##_8(which);     \
-    case 16: return sift_instance_##WG##_16(which);   \
-    case 32: return sift_instance_##WG##_32(which);   \
-    case 64: return sift_instance_##WG##_64(which);   \
-  }                                                   \
-  return nullptr;
-
-static void* instance(int which, int wg, int chunk) {
+static void* instance(int which, int wg) {
   switch (wg) {
-    case 64: { EMD_SWITCH_C(64, which, chunk) }
-    case 128: { EMD_SWITCH_C(128, which, chunk) }
-    case 256: { EMD_SWITCH_C(256, which, chunk) }
+    case 64: return sift_instance_64(which);
+    case 128: return sift_instance_128(which);
+    case 256: return sift_instance_256(which);
   }
   return nullptr;
 }
 
-static void* sift_kernel_ptr(int persistent, int global_tile, int wg, int chunk) {
-  return instance((persistent ? 1 : 0) + (global_tile ? 3 : 0), wg, chunk);
+static void* sift_kernel_ptr(int persistent, int global_tile, int wg) {
+  return instance((persistent ? 1 : 0) + (global_tile ? 3 : 0), wg);
 }
-static void* envelope_kernel_ptr(int global_tile, int wg, int chunk) {
-  return instance(global_tile ? 5 : 2, wg, chunk);
-}
-
-static size_t dyn_lds_bytes(int global_tile, const SiftParams& P) {
-  return global_tile ? 0 : (size_t)P.tile_slots * kSlotBytes;
+static void* envelope_kernel_ptr(int global_tile, int wg) {
+  return instance(global_tile ? 5 : 2, wg);
 }
 
-int launch_sift(int persistent, int global_tile, int wg, int chunk, int grid, const Batch& b,
+int launch_sift(int persistent, int global_tile, int wg, int grid, const Batch& b,
                 const SiftParams& P, hipStream_t stream) {
-  void* fn = sift_kernel_ptr(persistent, global_tile, wg, chunk);
+  void* fn = sift_kernel_ptr(persistent, global_tile, wg);
   if (!fn) return (int)hipErrorInvalidValue;
   if (grid <= 0) return (int)hipSuccess;
-  const size_t lds = dyn_lds_bytes(global_tile, P);
+  const size_t lds = dyn_lds_bytes(global_tile, P.tile_slots, P.max_n);
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return (int)e;
   Batch bb = b;
@@ -86,12 +57,12 @@ int launch_sift(int persistent, int global_tile, int wg, int chunk, int grid, co
   return (int)hipLaunchKernel(fn, dim3(grid), dim3(wg), args, lds, stream);
 }
 
-int launch_envelope(int global_tile, int wg, int chunk, const EnvBatch& e,
-                    const SiftParams& P, hipStream_t stream) {
-  void* fn = envelope_kernel_ptr(global_tile, wg, chunk);
+int launch_envelope(int global_tile, int wg, const EnvBatch& e, const SiftParams& P,
+                    hipStream_t stream) {
+  void* fn = envelope_kernel_ptr(global_tile, wg);
   if (!fn) return (int)hipErrorInvalidValue;
   if (e.B <= 0) return (int)hipSuccess;
-  const size_t lds = dyn_lds_bytes(global_tile, P);
+  const size_t lds = dyn_lds_bytes(global_tile, P.tile_slots, P.max_n);
   hipError_t r = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (r != hipSuccess) return (int)r;
   EnvBatch ee = e;
@@ -105,8 +76,8 @@ int reset_queue(unsigned* q, hipStream_t stream) {
   return (int)hipGetLastError();
 }
 
-int max_resident_wgs(int persistent, int global_tile, int wg, int chunk, size_t dyn_lds) {
-  void* fn = sift_kernel_ptr(persistent, global_tile, wg, chunk);
+int max_resident_wgs(int persistent, int global_tile, int wg, size_t dyn_lds) {
+  void* fn = sift_kernel_ptr(persistent, global_tile, wg);
   if (!fn) return 0;
   if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_lds) !=
       hipSuccess)

@@ -6,15 +6,17 @@
 namespace emdk {
 
 constexpr int kMaxNbsym = 4;      // mirrored points per side supported by the kernels
-constexpr int kMaxN = 16384;      // longest trace: 256 threads x 64 samples in registers
-constexpr int kSlotBytes = 44;    // LDS bytes per knot slot: 5 fp64 + 1 int32
+constexpr int kMaxN = 16384;      // longest trace (256 blocks of 64 samples)
+constexpr int kSlotBytes = 44;    // bytes per knot slot: 5 fp64 + 1 int32
+constexpr int kWorkBlockBytes = 568;  // LDS bytes per 64-sample block of the work area
 
 struct SiftParams {
   int max_extrema;    // per-side extrema cap (overflow rule); counts are clamped to it
   int nbsym;          // mirrored points per side (1..kMaxNbsym)
   int max_iteration;  // sift cap per IMF
   int max_imfs;       // output rows per trace (IMF cap = max_imfs - 1, then the residue)
-  int tile_slots;     // knot slots held in LDS (both envelopes)
+  int tile_slots;     // knot slots of the workgroup's tile (both envelopes)
+  int max_n;          // longest trace of the launch (sizes the LDS work area)
   int zero_fill;      // write zeros into output rows at and beyond n_imfs
 };
 
@@ -67,19 +69,20 @@ struct EnvBatch {
 int64_t tile_bytes(int slots);
 // Static LDS of the sift kernels for a workgroup size.
 int static_lds_bytes(int wg);
-// Smallest supported register chunk (samples per thread) for a trace length.
-int chunk_for(int n, int wg);
+// Dynamic LDS of a launch: the per-trace work area (IMF + block bookkeeping) for traces of
+// up to max_n samples, plus the knot tile unless it lives in global memory.
+size_t dyn_lds_bytes(int global_tile, int tile_slots, int max_n);
 
 // Launch one sift kernel instance.  persistent != 0 selects the queue kernel, whose grid
 // is `grid` workgroups; otherwise grid is the number of traces of the launch.  global_tile
 // selects the instance whose knot tile lives in global memory (b.spill).  Returns a
 // hipError_t.
-int launch_sift(int persistent, int global_tile, int wg, int chunk, int grid, const Batch& b,
+int launch_sift(int persistent, int global_tile, int wg, int grid, const Batch& b,
                 const SiftParams& P, hipStream_t stream);
-int launch_envelope(int global_tile, int wg, int chunk, const EnvBatch& e,
-                    const SiftParams& P, hipStream_t stream);
+int launch_envelope(int global_tile, int wg, const EnvBatch& e, const SiftParams& P,
+                    hipStream_t stream);
 int reset_queue(unsigned* q, hipStream_t stream);
 // Resident workgroups per CU for a kernel instance at the given dynamic LDS.
-int max_resident_wgs(int persistent, int global_tile, int wg, int chunk, size_t dyn_lds);
+int max_resident_wgs(int persistent, int global_tile, int wg, size_t dyn_lds);
 
 }  // namespace emdk

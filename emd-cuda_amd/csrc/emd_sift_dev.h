@@ -1,12 +1,14 @@
 // emd_sift_dev.h -- batched Empirical Mode Decomposition (PyEMD EMD() defaults) for gfx950.
 //
-// One workgroup decomposes one trace at a time.  The per-sample state (current IMF,
-// running sum of accepted IMFs) lives in VGPRs: thread t owns the contiguous chunk
-// [t*c, t*c+c) with c = next_pow2(ceil(N/WG)).  Only extrema-count-sized data lives in
-// LDS: the knot tile (positions, values, and the four tridiagonal arrays of the joint
-// max+min envelope solve).  A tile that does not fit the LDS budget of the launch spills
-// is deferred: the workgroup abandons the trace, appends it to a retry list, and a retry
-// launch with a larger tile (in LDS when it fits, else in global memory) redoes it.
+// One workgroup decomposes one trace at a time.  The current IMF lives in LDS (8 bytes per
+// sample) and is processed in blocks of 64 samples, one sample per lane: extrema are found
+// with wave64 ballots and compacted by popcount over a block-prefix scan, the same masks
+// give every sample its envelope interval, and every reduction is a 64-lane butterfly per
+// block followed by a perfect binary tree over the block sums.  The knot tile (positions,
+// values and the four tridiagonal arrays of the joint max+min envelope solve) follows the
+// IMF in LDS; a trace whose knots outgrow the launch's tile is deferred: the workgroup
+// abandons it, appends it to a retry list, and a retry launch with a tile that holds any
+// trace (in LDS when it fits, else in global memory) redoes it.
 //
 // Reference counterparts (SeismicFlow/emd-cuda kernels.py):
 //   detect()          <- find_extrema_serial / find_extrema_parallel   kernels.py:115-349
@@ -16,12 +18,12 @@
 //   decompose()       <- emd_batch_kernel body                          kernels.py:834-1080
 //   envelope kernel   <- envelope_batch_kernel                          kernels.py:1100-1166
 //
-// Canonical arithmetic: every floating-point reduction is the perfect binary tree over
-// next_pow2(N) samples in index order (in-thread pairwise tree, then the lane butterfly,
-// then waves in order), the tridiagonal solve's operation order depends only on row
-// indices, and the build uses -ffp-contract=off.  Results are therefore bit-identical for
-// every workgroup size, chunk size, LDS tile size, spill decision, queue order and kernel
-// (plain or persistent).
+// Canonical arithmetic: every floating-point sum is the perfect binary tree over
+// next_pow2(N) samples in index order (butterfly inside each aligned 64-sample block, then
+// a perfect tree over the blocks; zero leaves past N do not change it), the tridiagonal
+// solve's operation order depends only on row indices, and the build uses
+// -ffp-contract=off.  Results are therefore bit-identical for every workgroup size, LDS
+// tile, retry decision, queue order and kernel (plain or persistent).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -39,6 +41,7 @@ constexpr double kTotalPowerThr = 0.005;
 constexpr double kRangeThr = 0.001;
 constexpr double kPowerFloor = 1e-10;
 constexpr double kResidueAtol = 1e-8;
+constexpr int kWorkBytesPerBlock = 568;  // 64 fp64 + 4 fp64 + 2 u64 + 2 int per block
 
 enum SplineKind { kConst = 0, kLinear = 1, kThreePoint = 2, kNotAKnot = 3 };
 
@@ -65,24 +68,42 @@ __device__ __forceinline__ Tile make_tile(double* base, int slots) {
 template <int WG>
 struct SiftShared {
   static constexpr int NW = WG / 64;
-  double edge_first[WG];
-  double edge_last[WG];
   double red[NW][6];
-  double red2[NW][4];
-  unsigned scan_w[NW];
   int nz_w[NW];
   int plat_w[NW];
   int bad_w[NW];
   int app_w[NW];
-  double s_first, s_last;
-  // serial-path results
-  int ser_tm, ser_tn, ser_nm, ser_nn, ser_bad;
+  int tot_max, tot_min;
   // mirror results
   int k0[2], nk[2], kind[2], row0[2], mtot;
   int bad;
   double p3[2][4];
   int queue_idx;
 };
+
+// Per-trace LDS work area (ahead of the knot tile): the IMF and per-64-sample-block
+// bookkeeping, laid out for the launch's longest trace.
+struct Work {
+  double* imf;      // [nblk*64] current IMF
+  double* bsum;     // [4][nblk] block partial sums
+  uint64_t* mmax;   // [nblk] maxima masks of the last detection
+  uint64_t* mmin;   // [nblk] minima masks
+  int* pmax;        // [nblk] exclusive prefix counts of maxima over blocks
+  int* pmin;        // [nblk]
+  int nbcap;        // blocks of the layout
+};
+
+__device__ __forceinline__ Work make_work(double* base, int nbcap) {
+  Work w;
+  w.imf = base;
+  w.bsum = base + 64 * nbcap;
+  w.mmax = reinterpret_cast<uint64_t*>(base + 68 * nbcap);
+  w.mmin = w.mmax + nbcap;
+  w.pmax = reinterpret_cast<int*>(w.mmin + nbcap);
+  w.pmin = w.pmax + nbcap;
+  w.nbcap = nbcap;
+  return w;
+}
 
 __device__ __forceinline__ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 __device__ __forceinline__ int next_pow2(int x) {
@@ -138,45 +159,6 @@ __device__ __forceinline__ double waves_tree(const double* r, int stride) {
   }
 }
 
-// Streaming pairwise (perfect binary tree) accumulator over at most 2^(L-1) leaves pushed
-// in index order.  push<J>() merges with the stored left siblings given by J's trailing ones.
-template <int L>
-struct PairAcc {
-  double s[L];
-  template <int J>
-  __device__ __forceinline__ void push(double x) {
-    int lvl = 0;
-#pragma unroll
-    for (int jj = J; jj & 1; jj >>= 1) {
-      x = s[lvl] + x;
-      ++lvl;
-    }
-    s[lvl] = x;
-  }
-  // total of the first c leaves (c a power of two, 1 .. 2^(L-1))
-  __device__ __forceinline__ double total(int c) const {
-    double r = s[0];
-#pragma unroll
-    for (int l = 1; l < L; ++l)
-      if (c >= (1 << l)) r = s[l];
-    return r;
-  }
-};
-template <int C>
-constexpr int levels_for() {
-  int l = 1;
-  while ((1 << (l - 1)) < C) ++l;
-  return l;
-}
-
-template <int J, int C, class F>
-__device__ __forceinline__ void static_for(F&& f) {
-  if constexpr (J < C) {
-    f(std::integral_constant<int, J>{});
-    static_for<J + 1, C>(f);
-  }
-}
-
 // =======================================================================================
 // Extrema detection.  Semantics of PyEMD _find_extrema_simple: strict extrema where
 // d1*d2 < 0; plateau extrema at the half-to-even rounded midpoint of d==0 runs (first run
@@ -189,14 +171,17 @@ struct DetectResult {
   int tm, tn;      // true counts (unclamped)
   int nm, nn;      // counts used for the envelopes
   int nzer;
-  int defer;       // the knots (or the plateau staging copy) do not fit this launch's tile
+  int defer;       // the knots do not fit this launch's tile
   int ovf;
-  int exc_max, exc_min;  // this thread's exclusive offsets (-1 = unknown: serial path)
 };
 
-// Serial scan on one lane over a staged copy of the IMF: emit(idx, is_max) in index order.
+__device__ __forceinline__ uint64_t lanemask_lt(int lane) { return (uint64_t(1) << lane) - 1; }
+
+// Serial scan on one lane over a staged copy of the IMF: emit(idx) for every maximum
+// (want_max) or every minimum, in index order.
 template <class Emit>
-__device__ __forceinline__ void serial_extrema(const double* S, int N, Emit&& emit) {
+__device__ __forceinline__ void serial_extrema(const double* S, int N, bool want_max,
+                                               Emit&& emit) {
   if (N < 3) return;
   const double d_last = S[N - 1] - S[N - 2];  // d[-1]
   double dprev = 0.0;
@@ -204,8 +189,8 @@ __device__ __forceinline__ void serial_extrema(const double* S, int N, Emit&& em
   double run_dbefore = 0.0;
   bool first_run_seen = false;
   for (int k = 0; k <= N - 2; ++k) {
-    double dk = S[k + 1] - S[k];
-    if (k >= 1 && dprev * dk < 0.0) emit(k, dprev > 0.0);
+    const double dk = S[k + 1] - S[k];
+    if (k >= 1 && dprev * dk < 0.0 && (dprev > 0.0) == want_max) emit(k);
     if (dk == 0.0) {
       if (run_a < 0) {
         run_a = k;
@@ -219,9 +204,9 @@ __device__ __forceinline__ void serial_extrema(const double* S, int N, Emit&& em
       if (!(first && a == 1)) {
         const int mid = round_half_even_mid(a + b);
         if (run_dbefore > 0.0) {
-          if (dk < 0.0) emit(mid, true);
+          if (dk < 0.0 && want_max) emit(mid);
         } else {
-          if (dk > 0.0) emit(mid, false);
+          if (dk > 0.0 && !want_max) emit(mid);
         }
       }
     }
@@ -230,194 +215,127 @@ __device__ __forceinline__ void serial_extrema(const double* S, int N, Emit&& em
   // a run still open here reaches the end of the signal and is dropped
 }
 
-template <int WG, int C>
-__device__ __forceinline__ void write_knots_parallel(const Tile& t, const double (&v)[C], int cnt,
-                                                     int i0, uint64_t mmax, uint64_t mmin,
-                                                     int om, int on, int nm, int nn, int tb0,
-                                                     int tb1, int& bad) {
-  static_for<0, C>([&](auto jc) {
-    constexpr int j = decltype(jc)::value;
-    if (j < cnt) {
-      if ((mmax >> j) & 1) {
-        if (om < nm) {
-          t.X[tb0 + om] = i0 + j;
-          t.Y[tb0 + om] = v[j];
-          bad |= (v[j] < 0.0);
-        }
-        ++om;
-      }
-      if ((mmin >> j) & 1) {
-        if (on < nn) {
-          t.X[tb1 + on] = i0 + j;
-          t.Y[tb1 + on] = v[j];
-          bad |= (v[j] > 0.0);
-        }
-        ++on;
-      }
-    }
-  });
-}
-
-__device__ __forceinline__ void serial_write(const Tile& t, const double* S, int N, int nm, int nn, int tb0,
-                             int tb1, int& bad) {
-  int om = 0, on = 0;
-  serial_extrema(S, N, [&](int idx, bool is_max) {
-    double val = S[idx];
-    if (is_max) {
-      if (om < nm) {
-        t.X[tb0 + om] = idx;
-        t.Y[tb0 + om] = val;
-        bad |= (val < 0.0);
-      }
-      ++om;
-    } else {
-      if (on < nn) {
-        t.X[tb1 + on] = idx;
-        t.Y[tb1 + on] = val;
-        bad |= (val > 0.0);
-      }
-      ++on;
-    }
-  });
-}
-
-template <int WG, int C>
-__device__ __forceinline__ DetectResult detect(const double (&v)[C], int N, int c, int i0,
-                                               int cnt, SiftShared<WG>& sh, const Tile& t,
-                                               const SiftParams& P) {
+template <int WG>
+__device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared<WG>& sh,
+                                               const Tile& t, const SiftParams& P) {
   constexpr int NW = WG / 64;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int pad = P.nbsym;
+  const int nblk = ceil_div(N, 64);
+  const double* imf = wk.imf;
 
-  // ---- edge exchange ------------------------------------------------------------------
-  double last = v[0];
-  static_for<1, C>([&](auto jc) {
-    constexpr int j = decltype(jc)::value;
-    if (j < cnt) last = v[j];
-  });
-  if (cnt > 0) {
-    sh.edge_first[tid] = v[0];
-    sh.edge_last[tid] = last;
-    if (i0 == 0) sh.s_first = v[0];
-    if (i0 + cnt == N) sh.s_last = last;
-  }
-  __syncthreads();
-  const double prevE = (tid > 0 && cnt > 0) ? sh.edge_last[tid - 1] : 0.0;
-  const double nextE = (i0 + c < N) ? sh.edge_first[tid + 1] : 0.0;
-
-  // ---- local scan of the chunk --------------------------------------------------------
-  uint64_t mmax = 0, mmin = 0;
-  int nz = 0;
-  int plateau = 0;
-  static_for<0, C>([&](auto jc) {
-    constexpr int j = decltype(jc)::value;
-    if (j < cnt) {
-      const int i = i0 + j;
-      const double x = v[j];
-      double prev;
-      if constexpr (j == 0) prev = prevE; else prev = v[j - 1];
-      double next;
-      if constexpr (j + 1 < C) next = (j + 1 < cnt) ? v[j + 1] : nextE; else next = nextE;
+  // ---- pass A: per-block ballots of maxima/minima, zero crossings, plateaus ----------
+  int nz = 0, plat = 0;
+  for (int blk = w; blk < nblk; blk += NW) {
+    const int i = blk * 64 + lane;
+    bool is_max = false, is_min = false, zc = false, zr = false, pl = false;
+    if (i < N) {
+      const double x = imf[i];
+      const double prev = (i >= 1) ? imf[i - 1] : 0.0;
       if (i <= N - 2) {
+        const double next = imf[i + 1];
         const double d2 = next - x;
-        plateau |= (d2 == 0.0);
-        nz += (x * next < 0.0);
+        pl = (d2 == 0.0);
+        zc = (x * next < 0.0);
         if (i >= 1) {
           const double d1 = x - prev;
           const double p = d1 * d2;
-          if (p < 0.0) {
-            if (d1 > 0.0) mmax |= (uint64_t(1) << j);
-            if (d1 < 0.0) mmin |= (uint64_t(1) << j);
-          }
+          is_max = (p < 0.0) && (d1 > 0.0);
+          is_min = (p < 0.0) && (d1 < 0.0);
         }
       }
-      nz += (x == 0.0 && (i == 0 || prev != 0.0));
+      zr = (x == 0.0 && (i == 0 || prev != 0.0));
     }
-  });
-  const unsigned cm = __popcll(mmax);
-  const unsigned cn = __popcll(mmin);
-  const unsigned packed = cm | (cn << 16);
-  const unsigned incl = wave_incl_scan(packed, lane);
-  const int nzw = wave_sum_int(nz);
-  const int plw = (__ballot(plateau) != 0ull) ? 1 : 0;
-  if (lane == 63) sh.scan_w[w] = incl;
+    const uint64_t bm = __ballot(is_max), bn = __ballot(is_min);
+    nz += __popcll(__ballot(zc)) + __popcll(__ballot(zr));
+    plat |= (__ballot(pl) != 0ull);
+    if (lane == 0) {
+      wk.mmax[blk] = bm;
+      wk.mmin[blk] = bn;
+    }
+  }
   if (lane == 0) {
-    sh.nz_w[w] = nzw;
-    sh.plat_w[w] = plw;
+    sh.nz_w[w] = nz;
+    sh.plat_w[w] = plat;
   }
   __syncthreads();
-  unsigned pre = 0, tot = 0;
   int nzt = 0, plt = 0;
 #pragma unroll
   for (int q = 0; q < NW; ++q) {
-    const unsigned sw = sh.scan_w[q];
-    if (q < w) pre += sw;
-    tot += sw;
     nzt += sh.nz_w[q];
     plt |= sh.plat_w[q];
   }
-
+  if (plt) {
+    // plateau present: one lane applies the full run rules and rewrites the masks
+    if (tid == 0) {
+      for (int b = 0; b < nblk; ++b) {
+        wk.mmax[b] = 0ull;
+        wk.mmin[b] = 0ull;
+      }
+      serial_extrema(imf, N, true, [&](int idx) { wk.mmax[idx >> 6] |= uint64_t(1) << (idx & 63); });
+      serial_extrema(imf, N, false, [&](int idx) { wk.mmin[idx >> 6] |= uint64_t(1) << (idx & 63); });
+    }
+    __syncthreads();
+  }
+  // ---- exclusive prefix of per-block counts (wave 0; up to 4 blocks per lane) ---------
+  if (w == 0) {
+    const int per = ceil_div(nblk, 64);
+    unsigned loc = 0;
+    for (int k = 0; k < per; ++k) {
+      const int b = lane * per + k;
+      if (b < nblk) loc += __popcll(wk.mmax[b]) | (unsigned(__popcll(wk.mmin[b])) << 16);
+    }
+    const unsigned incl = wave_incl_scan(loc, lane);
+    unsigned run = incl - loc;
+    for (int k = 0; k < per; ++k) {
+      const int b = lane * per + k;
+      if (b < nblk) {
+        wk.pmax[b] = int(run & 0xffffu);
+        wk.pmin[b] = int(run >> 16);
+        run += __popcll(wk.mmax[b]) | (unsigned(__popcll(wk.mmin[b])) << 16);
+      }
+    }
+    if (lane == 63) {
+      sh.tot_max = int(incl & 0xffffu);
+      sh.tot_min = int(incl >> 16);
+    }
+  }
+  __syncthreads();
   DetectResult R;
   R.nzer = nzt;
-
-  if (plt) {
-    // ---- plateau present: one lane applies the full run rules to a copy of the IMF
-    // staged in the tile's tridiagonal arrays (4 fp64 per slot, free during detection)
-    R.exc_max = -1;
-    R.exc_min = -1;
-    if (N > 4 * P.tile_slots) {
-      R.defer = 1;
-      R.tm = R.tn = R.nm = R.nn = 0;
-      R.ovf = 0;
-      return R;
-    }
-    double* staging = t.A;
-    static_for<0, C>([&](auto jc) {
-      constexpr int j = decltype(jc)::value;
-      if (j < cnt) staging[i0 + j] = v[j];
-    });
-    __syncthreads();
-    if (tid == 0) {
-      int tm = 0, tn = 0;
-      serial_extrema(staging, N, [&](int, bool is_max) {
-        if (is_max) ++tm; else ++tn;
-      });
-      const int nm = min(tm, P.max_extrema), nn = min(tn, P.max_extrema);
-      int bad = 0;
-      if (4 * pad + nm + nn <= P.tile_slots)
-        serial_write(t, staging, N, nm, nn, pad, 3 * pad + nm, bad);
-      sh.ser_tm = tm;
-      sh.ser_tn = tn;
-      sh.ser_nm = nm;
-      sh.ser_nn = nn;
-      sh.ser_bad = bad;
-      sh.bad = bad;
-    }
-    __syncthreads();
-    R.tm = sh.ser_tm;
-    R.tn = sh.ser_tn;
-    R.nm = sh.ser_nm;
-    R.nn = sh.ser_nn;
-    R.defer = (4 * pad + R.nm + R.nn > P.tile_slots);
-    R.ovf = (R.tm >= P.max_extrema || R.tn >= P.max_extrema);
-    return R;
-  }
-
-  const unsigned excl = pre + incl - packed;
-  R.exc_max = int(excl & 0xffffu);
-  R.exc_min = int(excl >> 16);
-  R.tm = int(tot & 0xffffu);
-  R.tn = int(tot >> 16);
+  R.tm = sh.tot_max;
+  R.tn = sh.tot_min;
   R.nm = min(R.tm, P.max_extrema);
   R.nn = min(R.tn, P.max_extrema);
   R.ovf = (R.tm >= P.max_extrema || R.tn >= P.max_extrema);
   R.defer = (4 * pad + R.nm + R.nn > P.tile_slots);
   if (R.defer) return R;
+
+  // ---- pass B: compact the extrema into the knot tile ---------------------------------
+  const int tb0 = pad, tb1 = 3 * pad + R.nm;
   int bad = 0;
-  write_knots_parallel<WG, C>(t, v, cnt, i0, mmax, mmin, R.exc_max, R.exc_min, R.nm, R.nn, pad,
-                              3 * pad + R.nm, bad);
+  for (int blk = w; blk < nblk; blk += NW) {
+    const int i = blk * 64 + lane;
+    const uint64_t bm = wk.mmax[blk], bn = wk.mmin[blk];
+    if ((bm >> lane) & 1ull) {
+      const int k = wk.pmax[blk] + __popcll(bm & lanemask_lt(lane));
+      if (k < R.nm) {
+        const double y = imf[i];
+        t.X[tb0 + k] = i;
+        t.Y[tb0 + k] = y;
+        bad |= (y < 0.0);
+      }
+    }
+    if ((bn >> lane) & 1ull) {
+      const int k = wk.pmin[blk] + __popcll(bn & lanemask_lt(lane));
+      if (k < R.nn) {
+        const double y = imf[i];
+        t.X[tb1 + k] = i;
+        t.Y[tb1 + k] = y;
+        bad |= (y > 0.0);
+      }
+    }
+  }
   const int bw = (__ballot(bad) != 0ull) ? 1 : 0;
   if (lane == 0) sh.bad_w[w] = bw;
   __syncthreads();
@@ -714,14 +632,24 @@ __device__ __forceinline__ void spline_solve(const Tile& t, const SiftShared<WG>
       const double Mb = t.D[r0 + m - 1], Ma = t.D[r0 + m - 2];
       t.A[k0 + n - 1] = ((hnm3 + hnm2) * Mb - hnm2 * Ma) / hnm3;
     }
+    // per-interval constants by knot slot: Bd = 1/h, Cc = h*h/6 (Bd/Cc rows are dead)
+    for (int side = 0; side < 2; ++side) {
+      if (sh.kind[side] != kNotAKnot) continue;
+      const int k0 = sh.k0[side], n = sh.nk[side];
+      for (int k = tid; k < n - 1; k += WG) {
+        const double h = double(t.X[k0 + k + 1] - t.X[k0 + k]);
+        t.Bd[k0 + k] = 1.0 / h;
+        t.Cc[k0 + k] = h * h / 6.0;
+      }
+    }
   }
   __syncthreads();
 }
 
 // Envelope value at a sample for the closed-form spline kinds (3 knots: PyEMD's natural
 // cubic; 2 knots: straight line; 1 knot: constant).
-__device__ __noinline__ double eval_small(const Tile& t, int k0, int kind, const double* p3,
-                                          int pos) {
+__device__ __forceinline__ double eval_small(const Tile& t, int k0, int kind, const double* p3,
+                                             int pos) {
   const double tt = double(pos);
   const double X0 = t.X[k0], Y0 = t.Y[k0];
   if (kind == kConst) return Y0;
@@ -738,134 +666,116 @@ __device__ __noinline__ double eval_small(const Tile& t, int k0, int kind, const
   return w * Y1 + u * Y2 + u * w * (p3[2] * w + p3[3] * u);
 }
 
-// Cursor over one envelope's knots for increasing sample positions (not-a-knot kind keeps
-// the current interval in registers; the small kinds read their 2-3 knots per sample).
-struct EnvCursor {
-  int k0, nk, kind, j;
-  int xi0, xi1;
-  double y0, y1, M0, M1, invh, hh6;
-  const double* p3;
-  __device__ __forceinline__ void load(const Tile& t) {
-    xi0 = t.X[k0 + j];
-    xi1 = t.X[k0 + j + 1];
-    y0 = t.Y[k0 + j];
-    y1 = t.Y[k0 + j + 1];
-    M0 = t.A[k0 + j];
-    M1 = t.A[k0 + j + 1];
-    const double h = double(xi1 - xi0);
-    invh = 1.0 / h;
-    hh6 = h * h / 6.0;
-  }
-  __device__ __forceinline__ void init(const Tile& t, int k0_, int nk_, int kind_, int jstart,
-                                       const double* p3_) {
-    k0 = k0_;
-    nk = nk_;
-    kind = kind_;
-    p3 = p3_;
-    if (kind == kNotAKnot) {
-      j = max(0, min(jstart, nk - 2));
-      load(t);
-    }
-  }
-  __device__ __forceinline__ double eval(const Tile& t, int pos) {
-    if (kind != kNotAKnot) return eval_small(t, k0, kind, p3, pos);
-    bool moved = false;
-    while (j < nk - 2 && t.X[k0 + j + 1] <= pos) {
-      ++j;
-      moved = true;
-    }
-    while (j > 0 && t.X[k0 + j] > pos) {
-      --j;
-      moved = true;
-    }
-    if (moved) load(t);
-    const double A = double(xi1 - pos) * invh;
-    const double B = double(pos - xi0) * invh;
-    return A * y0 + B * y1 + ((A * A * A - A) * M0 + (B * B * B - B) * M1) * hh6;
-  }
-};
+// Interval of not-a-knot side `side` holding sample i (block blk, lane) from the masks of
+// the detection that produced the knots: true extrema at or before i plus the left knots,
+// corrected for left/right mirrored knots that straddle i (at most nbsym steps).
+template <int WG>
+__device__ __forceinline__ int interval_of(const Tile& t, const SiftShared<WG>& sh, int side,
+                                           const Work& wk, int blk, int lane, int i, int tb) {
+  const int k0 = sh.k0[side], nk = sh.nk[side];
+  const uint64_t m = side == 0 ? wk.mmax[blk] : wk.mmin[blk];
+  const int pre = side == 0 ? wk.pmax[blk] : wk.pmin[blk];
+  const int le = pre + __popcll(m & (lanemask_lt(lane) | (uint64_t(1) << lane)));
+  int j = (tb - k0) + le - 1;
+  j = max(0, min(j, nk - 2));
+  while (j < nk - 2 && t.X[k0 + j + 1] <= i) ++j;
+  while (j > 0 && t.X[k0 + j] > i) --j;
+  return j;
+}
 
 template <int WG>
-__device__ __forceinline__ int start_interval(const Tile& t, const SiftShared<WG>& sh, int side,
-                                              int exc, int i0, int tb) {
-  if (sh.kind[side] != kNotAKnot) return 0;
-  const int k0 = sh.k0[side], nk = sh.nk[side];
-  if (exc >= 0) return (tb - k0) + exc - 1;
-  // serial-path detection: binary search for the last knot at or before i0
-  int lo = 0, hi = nk - 2;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (t.X[k0 + mid] <= i0) lo = mid; else hi = mid - 1;
+__device__ __forceinline__ double envelope_at(const Tile& t, const SiftShared<WG>& sh, int side,
+                                              const Work& wk, int blk, int lane, int i, int tb) {
+  const int kind = sh.kind[side];
+  const int k0 = sh.k0[side];
+  if (kind != kNotAKnot) return eval_small(t, k0, kind, sh.p3[side], i);
+  const int j = interval_of(t, sh, side, wk, blk, lane, i, tb) + k0;
+  const double A = double(t.X[j + 1] - i) * t.Bd[j];
+  const double B = double(i - t.X[j]) * t.Bd[j];
+  return A * t.Y[j] + B * t.Y[j + 1] +
+         ((A * A * A - A) * t.A[j] + (B * B * B - B) * t.A[j + 1]) * t.Cc[j];
+}
+
+// Perfect binary tree over nblk block partials (zero-padded to a power of two), computed
+// identically by every lane of every wave.
+__device__ __forceinline__ double blocks_tree(const double* v, int nblk, int lane) {
+  int P = 1;
+  while (P < nblk) P <<= 1;
+  const int per = P > 64 ? P / 64 : 1;  // leaves per lane (1, 2 or 4)
+  double a = 0.0;
+  if (per == 1) {
+    a = lane < nblk ? v[lane] : 0.0;
+  } else if (per == 2) {
+    const int b = 2 * lane;
+    const double x0 = b < nblk ? v[b] : 0.0, x1 = b + 1 < nblk ? v[b + 1] : 0.0;
+    a = x0 + x1;
+  } else {
+    const int b = 4 * lane;
+    const double x0 = b < nblk ? v[b] : 0.0, x1 = b + 1 < nblk ? v[b + 1] : 0.0;
+    const double x2 = b + 2 < nblk ? v[b + 2] : 0.0, x3 = b + 3 < nblk ? v[b + 3] : 0.0;
+    a = (x0 + x1) + (x2 + x3);
   }
-  return lo;
+  // lanes past the leaves hold zeros: the 64-leaf tree equals the tree over P leaves
+  return wave_tree_sum(a);
 }
 
 struct SiftStats {
   double sq_new, sq_diff, sq_old, std_sum, old_max, old_min;
 };
 
-// One sift: evaluate both envelopes on the chunk, subtract the mean, accumulate the
-// PyEMD stop statistics.  v is updated in place.
-template <int WG, int C>
-__device__ __forceinline__ SiftStats envelope_update(const Tile& t, SiftShared<WG>& sh, double (&v)[C],
-                                     int c, int i0, int cnt, const DetectResult& D, int nb) {
+// One sift: evaluate both envelopes at every sample, subtract the mean, accumulate the
+// PyEMD stop statistics.  The IMF in LDS is updated in place.
+template <int WG>
+__device__ __forceinline__ SiftStats envelope_update(const Work& wk, const Tile& t,
+                                                     SiftShared<WG>& sh, int N,
+                                                     const DetectResult& D, int nb) {
   constexpr int NW = WG / 64;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  EnvCursor cmax, cmin;
+  const int nblk = ceil_div(N, 64);
   const int tb0 = nb, tb1 = 3 * nb + D.nm;
-  cmax.init(t, sh.k0[0], sh.nk[0], sh.kind[0], start_interval(t, sh, 0, D.exc_max, i0, tb0),
-            sh.p3[0]);
-  cmin.init(t, sh.k0[1], sh.nk[1], sh.kind[1], start_interval(t, sh, 1, D.exc_min, i0, tb1),
-            sh.p3[1]);
-  PairAcc<levels_for<C>()> a_new, a_diff, a_old, a_std;
   double omax = -INFINITY, omin = INFINITY;
-  static_for<0, C>([&](auto jc) {
-    constexpr int j = decltype(jc)::value;
-    if (j < c) {
-      double ln = 0.0, ld = 0.0, lo = 0.0, ls = 0.0;
-      if (j < cnt) {
-        const int pos = i0 + j;
-        const double emax = cmax.eval(t, pos);
-        const double emin = cmin.eval(t, pos);
-        const double mean = 0.5 * (emax + emin);
-        const double old = v[j];
-        const double nw = old - mean;
-        const double diff = nw - old;
-        const double r = diff / nw;
-        ln = nw * nw;
-        ld = diff * diff;
-        lo = old * old;
-        ls = r * r;
-        omax = fmax(omax, old);
-        omin = fmin(omin, old);
-        v[j] = nw;
-      }
-      a_new.template push<j>(ln);
-      a_diff.template push<j>(ld);
-      a_old.template push<j>(lo);
-      a_std.template push<j>(ls);
+  for (int blk = w; blk < nblk; blk += NW) {
+    const int i = blk * 64 + lane;
+    double ln = 0.0, ld = 0.0, lo = 0.0, ls = 0.0;
+    if (i < N) {
+      const double emax = envelope_at(t, sh, 0, wk, blk, lane, i, tb0);
+      const double emin = envelope_at(t, sh, 1, wk, blk, lane, i, tb1);
+      const double mean = 0.5 * (emax + emin);
+      const double old = wk.imf[i];
+      const double nw = old - mean;
+      const double diff = nw - old;
+      const double r = diff / nw;
+      ln = nw * nw;
+      ld = diff * diff;
+      lo = old * old;
+      ls = r * r;
+      omax = fmax(omax, old);
+      omin = fmin(omin, old);
+      wk.imf[i] = nw;
     }
-  });
-  double s0 = wave_tree_sum(a_new.total(c));
-  double s1 = wave_tree_sum(a_diff.total(c));
-  double s2 = wave_tree_sum(a_old.total(c));
-  double s3 = wave_tree_sum(a_std.total(c));
-  double mx = wave_max(omax);
-  double mn = wave_min(omin);
+    ln = wave_tree_sum(ln);
+    ld = wave_tree_sum(ld);
+    lo = wave_tree_sum(lo);
+    ls = wave_tree_sum(ls);
+    if (lane == 0) {
+      wk.bsum[blk] = ln;
+      wk.bsum[wk.nbcap + blk] = ld;
+      wk.bsum[2 * wk.nbcap + blk] = lo;
+      wk.bsum[3 * wk.nbcap + blk] = ls;
+    }
+  }
+  omax = wave_max(omax);
+  omin = wave_min(omin);
   if (lane == 0) {
-    sh.red[w][0] = s0;
-    sh.red[w][1] = s1;
-    sh.red[w][2] = s2;
-    sh.red[w][3] = s3;
-    sh.red[w][4] = mx;
-    sh.red[w][5] = mn;
+    sh.red[w][4] = omax;
+    sh.red[w][5] = omin;
   }
   __syncthreads();
   SiftStats st;
-  st.sq_new = waves_tree<NW>(&sh.red[0][0], 6);
-  st.sq_diff = waves_tree<NW>(&sh.red[0][1], 6);
-  st.sq_old = waves_tree<NW>(&sh.red[0][2], 6);
-  st.std_sum = waves_tree<NW>(&sh.red[0][3], 6);
+  st.sq_new = blocks_tree(wk.bsum, nblk, lane);
+  st.sq_diff = blocks_tree(wk.bsum + wk.nbcap, nblk, lane);
+  st.sq_old = blocks_tree(wk.bsum + 2 * wk.nbcap, nblk, lane);
+  st.std_sum = blocks_tree(wk.bsum + 3 * wk.nbcap, nblk, lane);
   double gmx = sh.red[0][4], gmn = sh.red[0][5];
 #pragma unroll
   for (int q = 1; q < NW; ++q) {
@@ -893,10 +803,10 @@ __device__ __forceinline__ bool check_imf(const SiftStats& s, int bad) {
 template <int WG>
 __device__ __forceinline__ void build_envelopes(const Tile& t, SiftShared<WG>& sh,
                                                 const DetectResult& D, int N,
-                                                const SiftParams& P) {
+                                                const SiftParams& P, const Work& wk) {
   if (threadIdx.x == 0) {
     int bad = sh.bad;
-    mirror(t, N, D.nm, D.nn, P.nbsym, sh.s_first, sh.s_last, sh.k0, sh.nk, sh.kind, bad,
+    mirror(t, N, D.nm, D.nn, P.nbsym, wk.imf[0], wk.imf[N - 1], sh.k0, sh.nk, sh.kind, bad,
            sh.p3);
     sh.bad = bad;
     const int m0 = sh.kind[0] == kNotAKnot ? sh.nk[0] - 2 : 0;
@@ -926,15 +836,14 @@ struct SignalIO {
 
 // Returns false when the trace was deferred (its knots outgrow this launch's tile); its
 // outputs are then left for the retry launch to write.
-template <int WG, int C>
+template <int WG>
 __device__ __forceinline__ bool decompose(const SignalIO& io, SiftShared<WG>& sh,
-                                          const Tile& t, const SiftParams& P, int flags0) {
+                                          const Work& wk, const Tile& t, const SiftParams& P,
+                                          int flags0) {
   constexpr int NW = WG / 64;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int N = io.N;
-  const int c = next_pow2(max(1, ceil_div(N, WG)));
-  const int i0 = tid * c;
-  const int cnt = max(0, min(N - i0, c));
+  const int nblk = ceil_div(N, 64);
   const int nb = P.nbsym;
 
   if (tid == 0) {
@@ -948,7 +857,6 @@ __device__ __forceinline__ bool decompose(const SignalIO& io, SiftShared<WG>& sh
   // is kept in the last output row, which no IMF occupies before the residue is written.
   double* acc = io.out + (int64_t)(P.max_imfs - 1) * N;
   bool have_acc = false;
-  double v[C];
 
   int imf_count = 0;
   bool finished = false;
@@ -957,10 +865,9 @@ __device__ __forceinline__ bool decompose(const SignalIO& io, SiftShared<WG>& sh
   int sifts = 0, peak = 0, knot_sum = 0, flags = flags0;
 
   while (!finished && imf_count < P.max_imfs - 1) {
-#pragma unroll
-    for (int j = 0; j < C; ++j)
-      v[j] = (j < cnt) ? io.S[i0 + j] - (have_acc ? acc[i0 + j] : 0.0) : 0.0;
-    DetectResult D = detect<WG, C>(v, N, c, i0, cnt, sh, t, P);
+    for (int i = tid; i < N; i += WG) wk.imf[i] = io.S[i] - (have_acc ? acc[i] : 0.0);
+    __syncthreads();
+    DetectResult D = detect<WG>(wk, N, sh, t, P);
     if (D.defer) return false;
     overflow |= D.ovf;
     peak = max(peak, max(D.tm, D.tn));
@@ -978,13 +885,13 @@ __device__ __forceinline__ bool decompose(const SignalIO& io, SiftShared<WG>& sh
       }
       const int extNo = D.nm + D.nn;
       if (extNo > 2 && D.nm >= 1 && D.nn >= 1) {
-        build_envelopes<WG>(t, sh, D, N, P);
-        const SiftStats st = envelope_update<WG, C>(t, sh, v, c, i0, cnt, D, nb);
+        build_envelopes<WG>(t, sh, D, N, P, wk);
+        const SiftStats st = envelope_update<WG>(wk, t, sh, N, D, nb);
         const int bad = sh.bad;
         ++sifts;
         knot_sum += sh.nk[0] + sh.nk[1];
-        __syncthreads();  // everyone has read sh.red / sh.bad / sh.nk before detect reuses them
-        D = detect<WG, C>(v, N, c, i0, cnt, sh, t, P);
+        __syncthreads();  // everyone has read sh/bsum before detect reuses them
+        D = detect<WG>(wk, N, sh, t, P);
         if (D.defer) return false;
         overflow |= D.ovf;
         peak = max(peak, max(D.tm, D.tn));
@@ -1010,39 +917,36 @@ __device__ __forceinline__ bool decompose(const SignalIO& io, SiftShared<WG>& sh
 
     // accept the IMF row, then PyEMD end_condition on S - (acc + imf)
     double* row = io.out + (int64_t)imf_count * N;
-    PairAcc<levels_for<C>()> a_abs;
     double tmax = -INFINITY, tmin = INFINITY;
-    static_for<0, C>([&](auto jc) {
-      constexpr int j = decltype(jc)::value;
-      if (j < c) {
-        double la = 0.0;
-        if (j < cnt) {
-          row[i0 + j] = v[j];
-          const double tmp = io.S[i0 + j] - ((have_acc ? acc[i0 + j] : 0.0) + v[j]);
-          la = fabs(tmp);
-          tmax = fmax(tmax, tmp);
-          tmin = fmin(tmin, tmp);
-        }
-        a_abs.template push<j>(la);
+    for (int blk = w; blk < nblk; blk += NW) {
+      const int i = blk * 64 + lane;
+      double la = 0.0;
+      if (i < N) {
+        const double v = wk.imf[i];
+        row[i] = v;
+        const double tmp = io.S[i] - ((have_acc ? acc[i] : 0.0) + v);
+        la = fabs(tmp);
+        tmax = fmax(tmax, tmp);
+        tmin = fmin(tmin, tmp);
       }
-    });
+      la = wave_tree_sum(la);
+      if (lane == 0) wk.bsum[blk] = la;
+    }
+    tmax = wave_max(tmax);
+    tmin = wave_min(tmin);
+    if (lane == 0) {
+      sh.red[w][0] = tmax;
+      sh.red[w][1] = tmin;
+    }
+    __syncthreads();
     {
-      const double sa = wave_tree_sum(a_abs.total(c));
-      const double mx = wave_max(tmax), mn = wave_min(tmin);
-      if (lane == 0) {
-        sh.red2[w][0] = sa;
-        sh.red2[w][1] = mx;
-        sh.red2[w][2] = mn;
-      }
-      __syncthreads();
-      const double gsa = waves_tree<NW>(&sh.red2[0][0], 4);
-      double gmx = sh.red2[0][1], gmn = sh.red2[0][2];
+      const double gsa = blocks_tree(wk.bsum, nblk, lane);
+      double gmx = sh.red[0][0], gmn = sh.red[0][1];
 #pragma unroll
       for (int q = 1; q < NW; ++q) {
-        gmx = fmax(gmx, sh.red2[q][1]);
-        gmn = fmin(gmn, sh.red2[q][2]);
+        gmx = fmax(gmx, sh.red[q][0]);
+        gmn = fmin(gmn, sh.red[q][1]);
       }
-      __syncthreads();
       if ((gmx - gmn) < kRangeThr) finished = true;
       else if (gsa < kTotalPowerThr) finished = true;
     }
@@ -1053,23 +957,19 @@ __device__ __forceinline__ bool decompose(const SignalIO& io, SiftShared<WG>& sh
     if (leaving && extNo_last <= 2) {
       --imf_count;  // the last row was a trend (PyEMD drops it); acc excludes it
     } else {
-#pragma unroll
-      for (int j = 0; j < C; ++j)
-        if (j < cnt) acc[i0 + j] = (have_acc ? acc[i0 + j] : 0.0) + v[j];
+      for (int i = tid; i < N; i += WG) acc[i] = (have_acc ? acc[i] : 0.0) + wk.imf[i];
       have_acc = true;
     }
+    __syncthreads();  // acc/bsum/red settled before the next IMF (or the residue) uses them
     if (leaving) break;  // PyEMD drops the trend row after leaving its loop
   }
 
   // residue = S - sum(IMFs); appended unless np.allclose(residue, 0) (max|r| <= 1e-8)
   int app = 0;
-#pragma unroll
-  for (int j = 0; j < C; ++j) {
-    if (j < cnt) {
-      const double r = io.S[i0 + j] - (have_acc ? acc[i0 + j] : 0.0);
-      v[j] = r;
-      app |= !(fabs(r) <= kResidueAtol);
-    }
+  for (int i = tid; i < N; i += WG) {
+    const double r = io.S[i] - (have_acc ? acc[i] : 0.0);
+    wk.imf[i] = r;
+    app |= !(fabs(r) <= kResidueAtol);
   }
   const int aw = (__ballot(app) != 0ull) ? 1 : 0;
   if (lane == 0) sh.app_w[w] = aw;
@@ -1080,17 +980,13 @@ __device__ __forceinline__ bool decompose(const SignalIO& io, SiftShared<WG>& sh
   int n_out = imf_count;
   if (append) {
     double* row = io.out + (int64_t)imf_count * N;
-#pragma unroll
-    for (int j = 0; j < C; ++j)
-      if (j < cnt) row[i0 + j] = v[j];
+    for (int i = tid; i < N; i += WG) row[i] = wk.imf[i];
     n_out = imf_count + 1;
   }
   if (P.zero_fill) {
     for (int k = n_out; k < P.max_imfs; ++k) {
       double* row = io.out + (int64_t)k * N;
-#pragma unroll
-      for (int j = 0; j < C; ++j)
-        if (j < cnt) row[i0 + j] = 0.0;
+      for (int i = tid; i < N; i += WG) row[i] = 0.0;
     }
   }
   if (tid == 0) {
@@ -1129,15 +1025,26 @@ __device__ __forceinline__ SignalIO signal_io(const Batch& b, int s, int max_imf
   return io;
 }
 
-// Knot tile of this workgroup: the dynamic LDS array, or (GT) a per-workgroup global region.
+// Dynamic LDS of a workgroup: the work area for traces up to P.max_n samples, then (for
+// LDS-tile instances) the knot tile.
+struct WgMem {
+  Work wk;
+  Tile t;
+};
+
 template <bool GT>
-__device__ __forceinline__ Tile workgroup_tile(double* lds, char* spill, int64_t stride,
-                                               int slots) {
+__device__ __forceinline__ WgMem workgroup_mem(double* lds, char* spill, int64_t stride,
+                                               const SiftParams& P) {
+  WgMem m;
+  const int nbcap = (P.max_n + 63) / 64;
+  m.wk = make_work(lds, nbcap);
   if constexpr (GT) {
-    return make_tile(reinterpret_cast<double*>(spill + (int64_t)blockIdx.x * stride), slots);
+    m.t = make_tile(reinterpret_cast<double*>(spill + (int64_t)blockIdx.x * stride),
+                    P.tile_slots);
   } else {
-    return make_tile(lds, slots);
+    m.t = make_tile(lds + (int64_t)nbcap * (kWorkBytesPerBlock / 8), P.tile_slots);
   }
+  return m;
 }
 
 __device__ __forceinline__ void defer_trace(const Batch& b, int s) {
@@ -1146,25 +1053,25 @@ __device__ __forceinline__ void defer_trace(const Batch& b, int s) {
 }
 
 // One workgroup per trace (grid = traces of this launch).  The reference launch model.
-template <int WG, int C, bool GT>
+template <int WG, bool GT>
 __global__ __launch_bounds__(WG) void emd_plain_kernel(Batch b, SiftParams P) {
   extern __shared__ double dyn_lds[];
   __shared__ SiftShared<WG> sh;
-  const Tile t = workgroup_tile<GT>(dyn_lds, b.spill, b.spill_stride, P.tile_slots);
+  const WgMem m = workgroup_mem<GT>(dyn_lds, b.spill, b.spill_stride, P);
   const int q = b.base + (int)blockIdx.x;
   const int s = b.order ? b.order[q] : q;
-  if (!decompose<WG, C>(signal_io(b, s, P.max_imfs), sh, t, P, b.retry ? 2 : 0) &&
+  if (!decompose<WG>(signal_io(b, s, P.max_imfs), sh, m.wk, m.t, P, b.retry ? 2 : 0) &&
       threadIdx.x == 0)
     defer_trace(b, s);
 }
 
 // Persistent workgroups pull queue positions from an atomic counter until the queue
 // (b.B entries, or *b.count_ptr when set) is drained.
-template <int WG, int C, bool GT>
+template <int WG, bool GT>
 __global__ __launch_bounds__(WG) void emd_persistent_kernel(Batch b, SiftParams P) {
   extern __shared__ double dyn_lds[];
   __shared__ SiftShared<WG> sh;
-  const Tile t = workgroup_tile<GT>(dyn_lds, b.spill, b.spill_stride, P.tile_slots);
+  const WgMem m = workgroup_mem<GT>(dyn_lds, b.spill, b.spill_stride, P);
   const int count = b.count_ptr ? (int)*b.count_ptr : b.B;
   while (true) {
     if (threadIdx.x == 0) sh.queue_idx = (int)atomicAdd(b.queue, 1u);
@@ -1174,29 +1081,27 @@ __global__ __launch_bounds__(WG) void emd_persistent_kernel(Batch b, SiftParams 
     if (idx >= count) break;
     const int q = b.base + idx;
     const int s = b.order ? b.order[q] : q;
-    if (!decompose<WG, C>(signal_io(b, s, P.max_imfs), sh, t, P, b.retry ? 2 : 0) &&
+    if (!decompose<WG>(signal_io(b, s, P.max_imfs), sh, m.wk, m.t, P, b.retry ? 2 : 0) &&
         threadIdx.x == 0)
       defer_trace(b, s);
   }
 }
 
 // Envelope pass without sifting (diagnostic; reference envelope_batch_kernel).
-template <int WG, int C, bool GT>
+template <int WG, bool GT>
 __global__ __launch_bounds__(WG) void emd_envelope_kernel(EnvBatch e, SiftParams P) {
   extern __shared__ double dyn_lds[];
   __shared__ SiftShared<WG> sh;
-  const int tid = threadIdx.x;
-  const Tile t = workgroup_tile<GT>(dyn_lds, e.spill, e.spill_stride, P.tile_slots);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  constexpr int NW = WG / 64;
+  const WgMem m = workgroup_mem<GT>(dyn_lds, e.spill, e.spill_stride, P);
+  const Tile& t = m.t;
   const int bidx = blockIdx.x;
   const int N = e.N;
   const double* S = e.values + (int64_t)bidx * N;
-  const int c = next_pow2(max(1, ceil_div(N, WG)));
-  const int i0 = tid * c;
-  const int cnt = max(0, min(N - i0, c));
-  double v[C];
-#pragma unroll
-  for (int j = 0; j < C; ++j) v[j] = (j < cnt) ? S[i0 + j] : 0.0;
-  DetectResult D = detect<WG, C>(v, N, c, i0, cnt, sh, t, P);
+  for (int i = tid; i < N; i += WG) m.wk.imf[i] = S[i];
+  __syncthreads();
+  DetectResult D = detect<WG>(m.wk, N, sh, t, P);
   if (D.defer) {
     if (tid == 0) e.degenerate[bidx] = -1;  // tile too small: caller retries with a larger one
     return;
@@ -1210,18 +1115,14 @@ __global__ __launch_bounds__(WG) void emd_envelope_kernel(EnvBatch e, SiftParams
     if (tid == 0) e.degenerate[bidx] = 1;
     return;
   }
-  build_envelopes<WG>(t, sh, D, N, P);
+  build_envelopes<WG>(t, sh, D, N, P, m.wk);
   const int nb = P.nbsym;
-  EnvCursor cmax, cmin;
-  cmax.init(t, sh.k0[0], sh.nk[0], sh.kind[0], start_interval(t, sh, 0, D.exc_max, i0, nb),
-            sh.p3[0]);
-  cmin.init(t, sh.k0[1], sh.nk[1], sh.kind[1],
-            start_interval(t, sh, 1, D.exc_min, i0, 3 * nb + D.nm), sh.p3[1]);
-#pragma unroll
-  for (int j = 0; j < C; ++j) {
-    if (j < cnt) {
-      e.max_env[(int64_t)bidx * N + i0 + j] = cmax.eval(t, i0 + j);
-      e.min_env[(int64_t)bidx * N + i0 + j] = cmin.eval(t, i0 + j);
+  const int nblk = ceil_div(N, 64);
+  for (int blk = w; blk < nblk; blk += NW) {
+    const int i = blk * 64 + lane;
+    if (i < N) {
+      e.max_env[(int64_t)bidx * N + i] = envelope_at(t, sh, 0, m.wk, blk, lane, i, nb);
+      e.min_env[(int64_t)bidx * N + i] = envelope_at(t, sh, 1, m.wk, blk, lane, i, 3 * nb + D.nm);
     }
   }
   for (int side = 0; side < 2; ++side) {
