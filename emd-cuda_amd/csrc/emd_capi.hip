@@ -39,13 +39,9 @@ struct emd_handle {
   int device = 0;
   int num_cus = 0;
   int lds_max = 0;
-  char* spill = nullptr;       // global knot tiles of the global-tile retry instances
+  char* spill = nullptr;       // per-workgroup global knot tiles
   size_t spill_size = 0;
-  int* defer_list = nullptr;   // traces deferred by the main launch
-  size_t defer_cap = 0;
-  unsigned* counters = nullptr;  // [0] queue head, [1] deferred count, [2] retry queue,
-                                 // [3] retry deferrals (always 0: the retry tile holds any
-                                 // trace of the launch)
+  unsigned* counters = nullptr;  // [0] persistent-kernel queue head
 };
 
 namespace {
@@ -71,32 +67,13 @@ int ensure_spill(emd_handle* h, size_t bytes) {
   return EMD_OK;
 }
 
-int ensure_defer(emd_handle* h, int B) {
-  if ((size_t)B <= h->defer_cap) return EMD_OK;
-  if (h->defer_list) {
-    hipError_t e = hipDeviceSynchronize();
-    if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize before growing the retry list");
-    (void)hipFree(h->defer_list);
-    h->defer_list = nullptr;
-    h->defer_cap = 0;
-  }
-  hipError_t e = hipMalloc(reinterpret_cast<void**>(&h->defer_list), (size_t)B * sizeof(int));
-  if (e != hipSuccess) {
-    h->defer_list = nullptr;
-    return fail(EMD_ERR_MEMORY, "hipMalloc(retry list of %d traces): %s", B,
-                hipGetErrorString(e));
-  }
-  h->defer_cap = (size_t)B;
-  return EMD_OK;
-}
-
 // knot slots that can never be exceeded for traces of length <= n
 int cap_slots(const emd_config& c, int n) {
   return 2 * std::min(c.max_extrema, std::max(n, 1)) + 4 * c.nbsym;
 }
 
 struct Shape {
-  int wg, tile, global_tile, max_n;
+  int wg, tile, spill_slots, max_n;
 };
 
 int check_wg(int wg) {
@@ -105,41 +82,30 @@ int check_wg(int wg) {
   return EMD_OK;
 }
 
-// knot slots that fit next to the work area of traces up to max_n samples
+// LDS knot slots that fit next to the work area of traces up to max_n samples
 int lds_fit_slots(const emd_handle* h, int wg, int max_n) {
-  const long avail = (long)h->lds_max - emdk::static_lds_bytes(wg) -
-                     (long)emdk::dyn_lds_bytes(1, 0, max_n);
+  const long avail =
+      (long)h->lds_max - emdk::static_lds_bytes(wg) - (long)emdk::dyn_lds_bytes(0, max_n);
   return avail > 0 ? (int)(avail / emdk::kSlotBytes) : 0;
 }
 
-// Main launch: tile_req slots (0 = the capacity rule), clamped to what LDS holds.
+// tile_req LDS knot slots (0 = the capacity rule), clamped to what LDS holds; a global
+// tile of the full capacity backs sifts whose knots do not fit.
 int make_shape(const emd_handle* h, int wg, int tile_req, int max_n, Shape* s) {
   int rc = check_wg(wg);
   if (rc) return rc;
   if (max_n > emdk::kMaxN)
     return fail(EMD_ERR_VALUE, "trace length %d exceeds the supported maximum %d", max_n,
                 emdk::kMaxN);
+  if (emdk::dyn_lds_bytes(0, max_n) + emdk::static_lds_bytes(wg) > (size_t)h->lds_max)
+    return fail(EMD_ERR_VALUE, "trace length %d does not fit the LDS work area", max_n);
   const int cap = cap_slots(h->cfg, max_n);
   int tile = tile_req > 0 ? std::min(tile_req, cap) : cap;
   tile = std::max(0, std::min(tile, lds_fit_slots(h, wg, max_n)));
   s->wg = wg;
   s->tile = tile;
-  s->global_tile = 0;
-  s->max_n = max_n;
-  if (emdk::dyn_lds_bytes(1, 0, max_n) + emdk::static_lds_bytes(wg) > (size_t)h->lds_max)
-    return fail(EMD_ERR_VALUE, "trace length %d does not fit the LDS work area", max_n);
-  return EMD_OK;
-}
-
-// Retry launch: a tile that holds every trace of the launch (LDS if it fits, else global).
-int make_retry_shape(const emd_handle* h, int wg, int max_n, Shape* s) {
-  int rc = make_shape(h, wg, 0, max_n, s);
-  if (rc) return rc;
-  const int cap = cap_slots(h->cfg, max_n);
-  if (cap > s->tile) {
-    s->tile = cap;
-    s->global_tile = 1;
-  }
+  s->spill_slots = tile < cap ? cap : 0;
+  s->max_n = std::max(max_n, 1);
   return EMD_OK;
 }
 
@@ -150,7 +116,8 @@ emdk::SiftParams params(const emd_handle* h, const Shape& s) {
   P.max_iteration = h->cfg.max_iteration;
   P.max_imfs = h->cfg.max_imfs;
   P.tile_slots = s.tile;
-  P.max_n = std::max(s.max_n, 1);
+  P.spill_slots = s.spill_slots;
+  P.max_n = s.max_n;
   P.zero_fill = 1;
   return P;
 }
@@ -237,48 +204,9 @@ int emd_destroy(emd_handle* h) {
   if (!h) return EMD_OK;
   hipError_t e = hipDeviceSynchronize();
   if (h->spill) (void)hipFree(h->spill);
-  if (h->defer_list) (void)hipFree(h->defer_list);
   if (h->counters) (void)hipFree(h->counters);
   delete h;
   if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize in emd_destroy");
-  return EMD_OK;
-}
-
-static int launch_shape(emd_handle* h, const Shape& s, int persistent, int per_cu_req, int B,
-                        emdk::Batch b, hipStream_t st, const char* what) {
-  const emdk::SiftParams P = params(h, s);
-  int rc;
-  if (s.global_tile) {
-    rc = ensure_spill(h, (size_t)emdk::tile_bytes(s.tile) *
-                             (size_t)(persistent ? h->num_cus * 4 : std::min(B, kMaxPlainBlocks)));
-    if (rc) return rc;
-    b.spill = h->spill;
-    b.spill_stride = emdk::tile_bytes(s.tile);
-  }
-  if (persistent) {
-    const size_t lds = emdk::dyn_lds_bytes(s.global_tile, s.tile, P.max_n);
-    const int fit = emdk::max_resident_wgs(1, s.global_tile, s.wg, lds);
-    if (fit <= 0) return fail(EMD_ERR_RUNTIME, "%s: kernel shape does not fit a CU", what);
-    int per_cu = per_cu_req;
-    if (per_cu <= 0 || per_cu > fit) per_cu = fit;
-    if (s.global_tile) per_cu = std::min(per_cu, 4);
-    int grid = h->num_cus * per_cu;
-    if (!b.count_ptr) grid = std::min(grid, B);
-    b.B = B;
-    b.base = 0;
-    rc = emdk::launch_sift(1, s.global_tile, s.wg, grid, b, P, st);
-    if (rc) return hip_fail((hipError_t)rc, what);
-    return check_launch(what);
-  }
-  for (int base = 0; base < B; base += kMaxPlainBlocks) {
-    const int nb = std::min(kMaxPlainBlocks, B - base);
-    b.B = nb;
-    b.base = base;
-    rc = emdk::launch_sift(0, s.global_tile, s.wg, nb, b, P, st);
-    if (rc) return hip_fail((hipError_t)rc, what);
-    rc = check_launch(what);
-    if (rc) return rc;
-  }
   return EMD_OK;
 }
 
@@ -296,17 +224,12 @@ static int run_common(emd_handle* h, const double* values, const int64_t* offset
   if ((max_n > 0 && (!values || !imfs)) || !n_imfs || !overflow || !debug_n || !debug_extno ||
       !debug_nzm)
     return fail(EMD_ERR_VALUE, "null input or output pointer");
-  Shape s, rs;
+  Shape s;
   int rc = make_shape(h, L.threads, L.tile_slots > 0 ? L.tile_slots : h->cfg.tile_slots,
                       max_n, &s);
   if (rc) return rc;
-  rc = make_retry_shape(h, s.wg, max_n, &rs);
-  if (rc) return rc;
-  rc = ensure_defer(h, B);
-  if (rc) return rc;
-  hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, st, h->counters);
-  rc = check_launch("counter reset");
-  if (rc) return rc;
+  const emdk::SiftParams P = params(h, s);
+  const int64_t stride = s.spill_slots ? emdk::tile_bytes(s.spill_slots) : 0;
   emdk::Batch b;
   b.values = values;
   b.offsets = offsets;
@@ -321,23 +244,44 @@ static int run_common(emd_handle* h, const double* values, const int64_t* offset
   b.dbg_z = debug_nzm;
   b.stats = stats;
   b.spill = nullptr;
-  b.spill_stride = 0;
+  b.spill_stride = stride;
   b.queue = h->counters;
-  b.count_ptr = nullptr;
-  b.defer_count = h->counters + 1;
-  b.defer_list = h->defer_list;
-  b.retry = 0;
-  rc = launch_shape(h, s, L.persistent, L.wgs_per_cu, B, b, st, "sift launch");
-  if (rc) return rc;
-  // Retry launch over the deferred traces (a device-side count; no host round trip).
-  emdk::Batch r = b;
-  r.order = h->defer_list;
-  r.queue = h->counters + 2;
-  r.count_ptr = h->counters + 1;
-  r.defer_count = h->counters + 3;
-  r.defer_list = h->defer_list;  // never written: the retry tile holds every trace
-  r.retry = 1;
-  return launch_shape(h, rs, 1, 0, B, r, st, "retry sift launch");
+  if (L.persistent) {
+    const size_t lds = emdk::dyn_lds_bytes(s.tile, P.max_n);
+    const int fit = emdk::max_resident_wgs(1, s.wg, lds);
+    if (fit <= 0) return fail(EMD_ERR_RUNTIME, "persistent kernel shape does not fit a CU");
+    int per_cu = L.wgs_per_cu;
+    if (per_cu <= 0 || per_cu > fit) per_cu = fit;
+    const int grid = std::min(B, h->num_cus * per_cu);
+    if (stride) {
+      rc = ensure_spill(h, (size_t)stride * grid);
+      if (rc) return rc;
+      b.spill = h->spill;
+    }
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, st, h->counters);
+    rc = check_launch("queue reset");
+    if (rc) return rc;
+    b.B = B;
+    b.base = 0;
+    rc = emdk::launch_sift(1, s.wg, grid, b, P, st);
+    if (rc) return hip_fail((hipError_t)rc, "persistent sift launch");
+    return check_launch("persistent sift kernel");
+  }
+  if (stride) {
+    rc = ensure_spill(h, (size_t)stride * std::min(B, kMaxPlainBlocks));
+    if (rc) return rc;
+    b.spill = h->spill;
+  }
+  for (int base = 0; base < B; base += kMaxPlainBlocks) {
+    const int nb = std::min(kMaxPlainBlocks, B - base);
+    b.B = nb;
+    b.base = base;
+    rc = emdk::launch_sift(0, s.wg, nb, b, P, st);
+    if (rc) return hip_fail((hipError_t)rc, "sift launch");
+    rc = check_launch("sift kernel");
+    if (rc) return rc;
+  }
+  return EMD_OK;
 }
 
 int emd_run_padded(emd_handle* h, const double* signals, int B, int N, double* imfs,
@@ -386,12 +330,13 @@ int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double
   if (B < 0) return fail(EMD_ERR_VALUE, "batch size must be >= 0, got %d", B);
   if (B == 0) return EMD_OK;
   Shape s;
-  int rc = make_retry_shape(h, 256, N, &s);  // a tile that holds every trace
+  int rc = make_shape(h, 256, h->cfg.tile_slots, N, &s);
   if (rc) return rc;
   const emdk::SiftParams P = params(h, s);
   const int mbuf = h->cfg.max_extrema + 2 * h->cfg.nbsym;
-  if (s.global_tile) {
-    rc = ensure_spill(h, (size_t)emdk::tile_bytes(s.tile) * std::min(B, kMaxPlainBlocks));
+  const int64_t stride = s.spill_slots ? emdk::tile_bytes(s.spill_slots) : 0;
+  if (stride) {
+    rc = ensure_spill(h, (size_t)stride * std::min(B, kMaxPlainBlocks));
     if (rc) return rc;
   }
   for (int base = 0; base < B; base += kMaxPlainBlocks) {
@@ -410,9 +355,9 @@ int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double
     e.min_kval = min_kval + (int64_t)base * mbuf;
     e.n_mirrored = n_mirrored + 2 * (int64_t)base;
     e.mbuf = mbuf;
-    e.spill = h->spill;
-    e.spill_stride = emdk::tile_bytes(s.tile);
-    rc = emdk::launch_envelope(s.global_tile, s.wg, e, P, (hipStream_t)stream);
+    e.spill = stride ? h->spill : nullptr;
+    e.spill_stride = stride;
+    rc = emdk::launch_envelope(s.wg, e, P, (hipStream_t)stream);
     if (rc) return hip_fail((hipError_t)rc, "envelope launch");
     rc = check_launch("envelope kernel");
     if (rc) return rc;
@@ -427,8 +372,8 @@ int emd_max_resident(emd_handle* h, const emd_launch* launch, int max_n, int* wg
                                                                  : h->cfg.tile_slots,
                       max_n, &s);
   if (rc) return rc;
-  *wgs_per_cu = emdk::max_resident_wgs(launch->persistent ? 1 : 0, 0, s.wg,
-                                       emdk::dyn_lds_bytes(0, s.tile, std::max(max_n, 1)));
+  *wgs_per_cu = emdk::max_resident_wgs(launch->persistent ? 1 : 0, s.wg,
+                                       emdk::dyn_lds_bytes(s.tile, s.max_n));
   return EMD_OK;
 }
 

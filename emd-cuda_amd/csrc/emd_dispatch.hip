@@ -22,9 +22,9 @@ int static_lds_bytes(int wg) {
   return 0;
 }
 
-size_t dyn_lds_bytes(int global_tile, int tile_slots, int max_n) {
+size_t dyn_lds_bytes(int tile_slots, int max_n) {
   const size_t nbcap = (size_t)(max_n + 63) / 64;
-  return nbcap * kWorkBlockBytes + (global_tile ? 0 : (size_t)tile_slots * kSlotBytes);
+  return nbcap * kWorkBlockBytes + (size_t)tile_slots * kSlotBytes;
 }
 
 static void* instance(int which, int wg) {
@@ -36,19 +36,15 @@ static void* instance(int which, int wg) {
   return nullptr;
 }
 
-static void* sift_kernel_ptr(int persistent, int global_tile, int wg) {
-  return instance((persistent ? 1 : 0) + (global_tile ? 3 : 0), wg);
-}
-static void* envelope_kernel_ptr(int global_tile, int wg) {
-  return instance(global_tile ? 5 : 2, wg);
-}
+static void* sift_kernel_ptr(int persistent, int wg) { return instance(persistent ? 1 : 0, wg); }
+static void* envelope_kernel_ptr(int wg) { return instance(2, wg); }
 
-int launch_sift(int persistent, int global_tile, int wg, int grid, const Batch& b,
-                const SiftParams& P, hipStream_t stream) {
-  void* fn = sift_kernel_ptr(persistent, global_tile, wg);
+int launch_sift(int persistent, int wg, int grid, const Batch& b, const SiftParams& P,
+                hipStream_t stream) {
+  void* fn = sift_kernel_ptr(persistent, wg);
   if (!fn) return (int)hipErrorInvalidValue;
   if (grid <= 0) return (int)hipSuccess;
-  const size_t lds = dyn_lds_bytes(global_tile, P.tile_slots, P.max_n);
+  const size_t lds = dyn_lds_bytes(P.tile_slots, P.max_n);
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return (int)e;
   Batch bb = b;
@@ -57,12 +53,11 @@ int launch_sift(int persistent, int global_tile, int wg, int grid, const Batch& 
   return (int)hipLaunchKernel(fn, dim3(grid), dim3(wg), args, lds, stream);
 }
 
-int launch_envelope(int global_tile, int wg, const EnvBatch& e, const SiftParams& P,
-                    hipStream_t stream) {
-  void* fn = envelope_kernel_ptr(global_tile, wg);
+int launch_envelope(int wg, const EnvBatch& e, const SiftParams& P, hipStream_t stream) {
+  void* fn = envelope_kernel_ptr(wg);
   if (!fn) return (int)hipErrorInvalidValue;
   if (e.B <= 0) return (int)hipSuccess;
-  const size_t lds = dyn_lds_bytes(global_tile, P.tile_slots, P.max_n);
+  const size_t lds = dyn_lds_bytes(P.tile_slots, P.max_n);
   hipError_t r = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (r != hipSuccess) return (int)r;
   EnvBatch ee = e;
@@ -76,8 +71,8 @@ int reset_queue(unsigned* q, hipStream_t stream) {
   return (int)hipGetLastError();
 }
 
-int max_resident_wgs(int persistent, int global_tile, int wg, size_t dyn_lds) {
-  void* fn = sift_kernel_ptr(persistent, global_tile, wg);
+int max_resident_wgs(int persistent, int wg, size_t dyn_lds) {
+  void* fn = sift_kernel_ptr(persistent, wg);
   if (!fn) return 0;
   if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_lds) !=
       hipSuccess)

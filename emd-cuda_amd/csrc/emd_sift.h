@@ -15,7 +15,8 @@ struct SiftParams {
   int nbsym;          // mirrored points per side (1..kMaxNbsym)
   int max_iteration;  // sift cap per IMF
   int max_imfs;       // output rows per trace (IMF cap = max_imfs - 1, then the residue)
-  int tile_slots;     // knot slots of the workgroup's tile (both envelopes)
+  int tile_slots;     // knot slots of the LDS tile (both envelopes)
+  int spill_slots;    // knot slots of each workgroup's global tile (holds any trace)
   int max_n;          // longest trace of the launch (sizes the LDS work area)
   int zero_fill;      // write zeros into output rows at and beyond n_imfs
 };
@@ -36,13 +37,9 @@ struct Batch {
   int* dbg_e;
   int* dbg_z;
   int* stats;
-  char* spill;                 // per-workgroup global knot tiles (global-tile instances)
+  char* spill;                 // per-workgroup global knot tiles
   int64_t spill_stride;        // bytes per workgroup
   unsigned* queue;             // persistent-kernel queue head
-  const unsigned* count_ptr;   // queue length in device memory (retry launches) or null
-  unsigned* defer_count;       // traces deferred by this launch (device counter)
-  int* defer_list;             // their indices
-  int retry;                   // 1 for a retry launch (sets stats flag bit 1)
 };
 
 // Envelope-only pass outputs (reference envelope_batch_kernel ABI, kernels.py:1100-1113).
@@ -70,19 +67,17 @@ int64_t tile_bytes(int slots);
 // Static LDS of the sift kernels for a workgroup size.
 int static_lds_bytes(int wg);
 // Dynamic LDS of a launch: the per-trace work area (IMF + block bookkeeping) for traces of
-// up to max_n samples, plus the knot tile unless it lives in global memory.
-size_t dyn_lds_bytes(int global_tile, int tile_slots, int max_n);
+// up to max_n samples, plus the LDS knot tile.
+size_t dyn_lds_bytes(int tile_slots, int max_n);
 
 // Launch one sift kernel instance.  persistent != 0 selects the queue kernel, whose grid
-// is `grid` workgroups; otherwise grid is the number of traces of the launch.  global_tile
-// selects the instance whose knot tile lives in global memory (b.spill).  Returns a
+// is `grid` workgroups; otherwise grid is the number of traces of the launch.  Returns a
 // hipError_t.
-int launch_sift(int persistent, int global_tile, int wg, int grid, const Batch& b,
-                const SiftParams& P, hipStream_t stream);
-int launch_envelope(int global_tile, int wg, const EnvBatch& e, const SiftParams& P,
-                    hipStream_t stream);
+int launch_sift(int persistent, int wg, int grid, const Batch& b, const SiftParams& P,
+                hipStream_t stream);
+int launch_envelope(int wg, const EnvBatch& e, const SiftParams& P, hipStream_t stream);
 int reset_queue(unsigned* q, hipStream_t stream);
 // Resident workgroups per CU for a kernel instance at the given dynamic LDS.
-int max_resident_wgs(int persistent, int global_tile, int wg, size_t dyn_lds);
+int max_resident_wgs(int persistent, int wg, size_t dyn_lds);
 
 }  // namespace emdk

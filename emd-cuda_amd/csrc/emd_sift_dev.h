@@ -6,9 +6,8 @@
 // give every sample its envelope interval, and every reduction is a 64-lane butterfly per
 // block followed by a perfect binary tree over the block sums.  The knot tile (positions,
 // values and the four tridiagonal arrays of the joint max+min envelope solve) follows the
-// IMF in LDS; a trace whose knots outgrow the launch's tile is deferred: the workgroup
-// abandons it, appends it to a retry list, and a retry launch with a tile that holds any
-// trace (in LDS when it fits, else in global memory) redoes it.
+// IMF in LDS; a sift whose knots outgrow the launch's LDS tile uses the workgroup's own
+// tile region in global memory instead (same code, same arithmetic).
 //
 // Reference counterparts (SeismicFlow/emd-cuda kernels.py):
 //   detect()          <- find_extrema_serial / find_extrema_parallel   kernels.py:115-349
@@ -23,7 +22,7 @@
 // a perfect tree over the blocks; zero leaves past N do not change it), the tridiagonal
 // solve's operation order depends only on row indices, and the build uses
 // -ffp-contract=off.  Results are therefore bit-identical for every workgroup size, LDS
-// tile, retry decision, queue order and kernel (plain or persistent).
+// tile, tile placement, queue order and kernel (plain or persistent).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -106,6 +105,9 @@ __device__ __forceinline__ Work make_work(double* base, int nbcap) {
 }
 
 __device__ __forceinline__ int ceil_div(int a, int b) { return (a + b - 1) / b; }
+// Values that are equal in every lane by construction (read back from LDS after a
+// barrier): make them scalar so branches on them are uniform jumps.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ int next_pow2(int x) {
   int p = 1;
   while (p < x) p <<= 1;
@@ -171,7 +173,7 @@ struct DetectResult {
   int tm, tn;      // true counts (unclamped)
   int nm, nn;      // counts used for the envelopes
   int nzer;
-  int defer;       // the knots do not fit this launch's tile
+  int in_lds;      // the knots of this detection are in the LDS tile (else the global one)
   int ovf;
 };
 
@@ -216,8 +218,40 @@ __device__ __forceinline__ void serial_extrema(const double* S, int N, bool want
 }
 
 template <int WG>
+__device__ __forceinline__ void compact_knots(const Work& wk, int N, const Tile& t, int nm,
+                                              int nn, int pad, int& bad) {
+  constexpr int NW = WG / 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nblk = ceil_div(N, 64);
+  const int tb0 = pad, tb1 = 3 * pad + nm;
+  for (int blk = w; blk < nblk; blk += NW) {
+    const int i = blk * 64 + lane;
+    const uint64_t bm = wk.mmax[blk], bn = wk.mmin[blk];
+    if ((bm >> lane) & 1ull) {
+      const int k = wk.pmax[blk] + __popcll(bm & lanemask_lt(lane));
+      if (k < nm) {
+        const double y = wk.imf[i];
+        t.X[tb0 + k] = i;
+        t.Y[tb0 + k] = y;
+        bad |= (y < 0.0);
+      }
+    }
+    if ((bn >> lane) & 1ull) {
+      const int k = wk.pmin[blk] + __popcll(bn & lanemask_lt(lane));
+      if (k < nn) {
+        const double y = wk.imf[i];
+        t.X[tb1 + k] = i;
+        t.Y[tb1 + k] = y;
+        bad |= (y > 0.0);
+      }
+    }
+  }
+}
+
+template <int WG>
 __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared<WG>& sh,
-                                               const Tile& t, const SiftParams& P) {
+                                               const Tile& tl, const Tile& tg,
+                                               const SiftParams& P) {
   constexpr int NW = WG / 64;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int pad = P.nbsym;
@@ -265,7 +299,7 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
     nzt += sh.nz_w[q];
     plt |= sh.plat_w[q];
   }
-  if (plt) {
+  if (uni(plt)) {
     // plateau present: one lane applies the full run rules and rewrites the masks
     if (tid == 0) {
       for (int b = 0; b < nblk; ++b) {
@@ -302,40 +336,18 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
   }
   __syncthreads();
   DetectResult R;
-  R.nzer = nzt;
-  R.tm = sh.tot_max;
-  R.tn = sh.tot_min;
+  R.nzer = uni(nzt);
+  R.tm = uni(sh.tot_max);
+  R.tn = uni(sh.tot_min);
   R.nm = min(R.tm, P.max_extrema);
   R.nn = min(R.tn, P.max_extrema);
   R.ovf = (R.tm >= P.max_extrema || R.tn >= P.max_extrema);
-  R.defer = (4 * pad + R.nm + R.nn > P.tile_slots);
-  if (R.defer) return R;
+  R.in_lds = (4 * pad + R.nm + R.nn <= P.tile_slots);
 
   // ---- pass B: compact the extrema into the knot tile ---------------------------------
-  const int tb0 = pad, tb1 = 3 * pad + R.nm;
   int bad = 0;
-  for (int blk = w; blk < nblk; blk += NW) {
-    const int i = blk * 64 + lane;
-    const uint64_t bm = wk.mmax[blk], bn = wk.mmin[blk];
-    if ((bm >> lane) & 1ull) {
-      const int k = wk.pmax[blk] + __popcll(bm & lanemask_lt(lane));
-      if (k < R.nm) {
-        const double y = imf[i];
-        t.X[tb0 + k] = i;
-        t.Y[tb0 + k] = y;
-        bad |= (y < 0.0);
-      }
-    }
-    if ((bn >> lane) & 1ull) {
-      const int k = wk.pmin[blk] + __popcll(bn & lanemask_lt(lane));
-      if (k < R.nn) {
-        const double y = imf[i];
-        t.X[tb1 + k] = i;
-        t.Y[tb1 + k] = y;
-        bad |= (y > 0.0);
-      }
-    }
-  }
+  if (R.in_lds) compact_knots<WG>(wk, N, tl, R.nm, R.nn, pad, bad);
+  else compact_knots<WG>(wk, N, tg, R.nm, R.nn, pad, bad);
   const int bw = (__ballot(bad) != 0ull) ? 1 : 0;
   if (lane == 0) sh.bad_w[w] = bw;
   __syncthreads();
@@ -543,8 +555,8 @@ __device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int
 template <int WG>
 __device__ __forceinline__ void spline_solve(const Tile& t, const SiftShared<WG>& sh) {
   const int tid = threadIdx.x;
-  const int mtot = sh.mtot;
-  const int row1 = sh.row0[1];
+  const int mtot = uni(sh.mtot);
+  const int row1 = uni(sh.row0[1]);
   for (int r = tid; r < mtot; r += WG) {
     const int side = (r < row1) ? 0 : 1;
     const int k = r - sh.row0[side];
@@ -834,12 +846,11 @@ struct SignalIO {
   int* stats;       // 4 ints or null
 };
 
-// Returns false when the trace was deferred (its knots outgrow this launch's tile); its
-// outputs are then left for the retry launch to write.
+// tl: the launch's LDS knot tile; tg: this workgroup's global tile (holds any trace).
 template <int WG>
-__device__ __forceinline__ bool decompose(const SignalIO& io, SiftShared<WG>& sh,
-                                          const Work& wk, const Tile& t, const SiftParams& P,
-                                          int flags0) {
+__device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh,
+                                          const Work& wk, const Tile& tl, const Tile& tg,
+                                          const SiftParams& P) {
   constexpr int NW = WG / 64;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int N = io.N;
@@ -862,13 +873,12 @@ __device__ __forceinline__ bool decompose(const SignalIO& io, SiftShared<WG>& sh
   bool finished = false;
   int extNo_last = -1;
   int overflow = 0;
-  int sifts = 0, peak = 0, knot_sum = 0, flags = flags0;
+  int sifts = 0, peak = 0, knot_sum = 0, flags = 0;
 
   while (!finished && imf_count < P.max_imfs - 1) {
     for (int i = tid; i < N; i += WG) wk.imf[i] = io.S[i] - (have_acc ? acc[i] : 0.0);
     __syncthreads();
-    DetectResult D = detect<WG>(wk, N, sh, t, P);
-    if (D.defer) return false;
+    DetectResult D = detect<WG>(wk, N, sh, tl, tg, P);
     overflow |= D.ovf;
     peak = max(peak, max(D.tm, D.tn));
     int n = 0;
@@ -885,20 +895,26 @@ __device__ __forceinline__ bool decompose(const SignalIO& io, SiftShared<WG>& sh
       }
       const int extNo = D.nm + D.nn;
       if (extNo > 2 && D.nm >= 1 && D.nn >= 1) {
-        build_envelopes<WG>(t, sh, D, N, P, wk);
-        const SiftStats st = envelope_update<WG>(wk, t, sh, N, D, nb);
-        const int bad = sh.bad;
+        SiftStats st;
+        if (D.in_lds) {
+          build_envelopes<WG>(tl, sh, D, N, P, wk);
+          st = envelope_update<WG>(wk, tl, sh, N, D, nb);
+        } else {
+          flags |= 2;
+          build_envelopes<WG>(tg, sh, D, N, P, wk);
+          st = envelope_update<WG>(wk, tg, sh, N, D, nb);
+        }
+        const int bad = uni(sh.bad);
         ++sifts;
-        knot_sum += sh.nk[0] + sh.nk[1];
+        knot_sum += uni(sh.nk[0] + sh.nk[1]);
         __syncthreads();  // everyone has read sh/bsum before detect reuses them
-        D = detect<WG>(wk, N, sh, t, P);
-        if (D.defer) return false;
+        D = detect<WG>(wk, N, sh, tl, tg, P);
         overflow |= D.ovf;
         peak = max(peak, max(D.tm, D.tn));
         const int extNo2 = D.nm + D.nn;
         const int nzm2 = D.nzer;
         const bool f2 = abs(extNo2 - nzm2) < 2;
-        const bool f1 = check_imf(st, bad);
+        const bool f1 = uni(check_imf(st, bad) ? 1 : 0) != 0;
         extNo_last = extNo2;
         if (f1 && f2) {
           if (tid == 0) {
@@ -947,8 +963,8 @@ __device__ __forceinline__ bool decompose(const SignalIO& io, SiftShared<WG>& sh
         gmx = fmax(gmx, sh.red[q][0]);
         gmn = fmin(gmn, sh.red[q][1]);
       }
-      if ((gmx - gmn) < kRangeThr) finished = true;
-      else if (gsa < kTotalPowerThr) finished = true;
+      const int done = ((gmx - gmn) < kRangeThr || gsa < kTotalPowerThr) ? 1 : 0;
+      if (uni(done)) finished = true;
     }
     ++imf_count;
     const bool capped = imf_count >= P.max_imfs - 1;
@@ -977,6 +993,7 @@ __device__ __forceinline__ bool decompose(const SignalIO& io, SiftShared<WG>& sh
   int append = 0;
 #pragma unroll
   for (int q = 0; q < NW; ++q) append |= sh.app_w[q];
+  append = uni(append);
   int n_out = imf_count;
   if (append) {
     double* row = io.out + (int64_t)imf_count * N;
@@ -1000,7 +1017,6 @@ __device__ __forceinline__ bool decompose(const SignalIO& io, SiftShared<WG>& sh
     }
   }
   __syncthreads();
-  return true;
 }
 
 __device__ __forceinline__ SignalIO signal_io(const Batch& b, int s, int max_imfs) {
@@ -1025,87 +1041,68 @@ __device__ __forceinline__ SignalIO signal_io(const Batch& b, int s, int max_imf
   return io;
 }
 
-// Dynamic LDS of a workgroup: the work area for traces up to P.max_n samples, then (for
-// LDS-tile instances) the knot tile.
+// Memory of a workgroup: the LDS work area for traces up to P.max_n samples, the LDS knot
+// tile after it, and the workgroup's global knot tile (P.spill_slots slots).
 struct WgMem {
   Work wk;
-  Tile t;
+  Tile tl, tg;
 };
 
-template <bool GT>
 __device__ __forceinline__ WgMem workgroup_mem(double* lds, char* spill, int64_t stride,
                                                const SiftParams& P) {
   WgMem m;
   const int nbcap = (P.max_n + 63) / 64;
   m.wk = make_work(lds, nbcap);
-  if constexpr (GT) {
-    m.t = make_tile(reinterpret_cast<double*>(spill + (int64_t)blockIdx.x * stride),
-                    P.tile_slots);
-  } else {
-    m.t = make_tile(lds + (int64_t)nbcap * (kWorkBytesPerBlock / 8), P.tile_slots);
-  }
+  m.tl = make_tile(lds + (int64_t)nbcap * (kWorkBytesPerBlock / 8), P.tile_slots);
+  m.tg = make_tile(reinterpret_cast<double*>(spill + (int64_t)blockIdx.x * stride),
+                   P.spill_slots);
   return m;
 }
 
-__device__ __forceinline__ void defer_trace(const Batch& b, int s) {
-  const unsigned k = atomicAdd(b.defer_count, 1u);
-  b.defer_list[k] = s;
-}
-
 // One workgroup per trace (grid = traces of this launch).  The reference launch model.
-template <int WG, bool GT>
+template <int WG>
 __global__ __launch_bounds__(WG) void emd_plain_kernel(Batch b, SiftParams P) {
   extern __shared__ double dyn_lds[];
   __shared__ SiftShared<WG> sh;
-  const WgMem m = workgroup_mem<GT>(dyn_lds, b.spill, b.spill_stride, P);
+  const WgMem m = workgroup_mem(dyn_lds, b.spill, b.spill_stride, P);
   const int q = b.base + (int)blockIdx.x;
   const int s = b.order ? b.order[q] : q;
-  if (!decompose<WG>(signal_io(b, s, P.max_imfs), sh, m.wk, m.t, P, b.retry ? 2 : 0) &&
-      threadIdx.x == 0)
-    defer_trace(b, s);
+  decompose<WG>(signal_io(b, s, P.max_imfs), sh, m.wk, m.tl, m.tg, P);
 }
 
-// Persistent workgroups pull queue positions from an atomic counter until the queue
-// (b.B entries, or *b.count_ptr when set) is drained.
-template <int WG, bool GT>
+// Persistent workgroups pull queue positions from an atomic counter until the b.B queue
+// entries are drained.
+template <int WG>
 __global__ __launch_bounds__(WG) void emd_persistent_kernel(Batch b, SiftParams P) {
   extern __shared__ double dyn_lds[];
   __shared__ SiftShared<WG> sh;
-  const WgMem m = workgroup_mem<GT>(dyn_lds, b.spill, b.spill_stride, P);
-  const int count = b.count_ptr ? (int)*b.count_ptr : b.B;
+  const WgMem m = workgroup_mem(dyn_lds, b.spill, b.spill_stride, P);
   while (true) {
     if (threadIdx.x == 0) sh.queue_idx = (int)atomicAdd(b.queue, 1u);
     __syncthreads();
-    const int idx = sh.queue_idx;
+    const int idx = __builtin_amdgcn_readfirstlane(sh.queue_idx);
     __syncthreads();
-    if (idx >= count) break;
+    if (idx >= b.B) break;
     const int q = b.base + idx;
     const int s = b.order ? b.order[q] : q;
-    if (!decompose<WG>(signal_io(b, s, P.max_imfs), sh, m.wk, m.t, P, b.retry ? 2 : 0) &&
-        threadIdx.x == 0)
-      defer_trace(b, s);
+    decompose<WG>(signal_io(b, s, P.max_imfs), sh, m.wk, m.tl, m.tg, P);
   }
 }
 
 // Envelope pass without sifting (diagnostic; reference envelope_batch_kernel).
-template <int WG, bool GT>
+template <int WG>
 __global__ __launch_bounds__(WG) void emd_envelope_kernel(EnvBatch e, SiftParams P) {
   extern __shared__ double dyn_lds[];
   __shared__ SiftShared<WG> sh;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   constexpr int NW = WG / 64;
-  const WgMem m = workgroup_mem<GT>(dyn_lds, e.spill, e.spill_stride, P);
-  const Tile& t = m.t;
+  const WgMem m = workgroup_mem(dyn_lds, e.spill, e.spill_stride, P);
   const int bidx = blockIdx.x;
   const int N = e.N;
   const double* S = e.values + (int64_t)bidx * N;
   for (int i = tid; i < N; i += WG) m.wk.imf[i] = S[i];
   __syncthreads();
-  DetectResult D = detect<WG>(m.wk, N, sh, t, P);
-  if (D.defer) {
-    if (tid == 0) e.degenerate[bidx] = -1;  // tile too small: caller retries with a larger one
-    return;
-  }
+  DetectResult D = detect<WG>(m.wk, N, sh, m.tl, m.tg, P);
   if (tid == 0) {
     e.n_extrema[2 * bidx] = D.nm;
     e.n_extrema[2 * bidx + 1] = D.nn;
@@ -1115,7 +1112,9 @@ __global__ __launch_bounds__(WG) void emd_envelope_kernel(EnvBatch e, SiftParams
     if (tid == 0) e.degenerate[bidx] = 1;
     return;
   }
-  build_envelopes<WG>(t, sh, D, N, P, m.wk);
+  const Tile& t = D.in_lds ? m.tl : m.tg;
+  if (D.in_lds) build_envelopes<WG>(m.tl, sh, D, N, P, m.wk);
+  else build_envelopes<WG>(m.tg, sh, D, N, P, m.wk);
   const int nb = P.nbsym;
   const int nblk = ceil_div(N, 64);
   for (int blk = w; blk < nblk; blk += NW) {

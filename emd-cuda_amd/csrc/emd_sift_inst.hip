@@ -12,16 +12,12 @@
 #define EMD_INST_FN EMD_PASTE(sift_instance_, EMD_WG)
 
 namespace emdk {
-// which: 0 plain, 1 persistent, 2 envelope (knot tile in LDS); 3, 4, 5 the same kernels
-// with the knot tile in global memory.
+// which: 0 plain, 1 persistent, 2 envelope
 void* EMD_INST_FN(int which) {
   switch (which) {
-    case 0: return reinterpret_cast<void*>(&emd_plain_kernel<EMD_WG, false>);
-    case 1: return reinterpret_cast<void*>(&emd_persistent_kernel<EMD_WG, false>);
-    case 2: return reinterpret_cast<void*>(&emd_envelope_kernel<EMD_WG, false>);
-    case 3: return reinterpret_cast<void*>(&emd_plain_kernel<EMD_WG, true>);
-    case 4: return reinterpret_cast<void*>(&emd_persistent_kernel<EMD_WG, true>);
-    case 5: return reinterpret_cast<void*>(&emd_envelope_kernel<EMD_WG, true>);
+    case 0: return reinterpret_cast<void*>(&emd_plain_kernel<EMD_WG>);
+    case 1: return reinterpret_cast<void*>(&emd_persistent_kernel<EMD_WG>);
+    case 2: return reinterpret_cast<void*>(&emd_envelope_kernel<EMD_WG>);
   }
   return nullptr;
 }

@@ -32,8 +32,8 @@ class EMDKernelConfig:
     ``max_extrema`` is the per-side extrema cap: counts are clamped to it and a trace whose
     count reaches it is flagged in ``overflow``.  ``tile_slots`` (addition) is the LDS knot
     tile of the one-workgroup-per-trace launch; 0 sizes it from ``max_extrema`` like the
-    reference's shared-memory budget.  Traces that outgrow the tile are redone by a retry
-    launch with a tile that holds them, so the tile never changes results.
+    reference's shared-memory budget.  A sift whose knots outgrow the LDS tile keeps them
+    in the workgroup's global tile instead, so the tile never changes results.
     """
 
     max_extrema: int = 1024

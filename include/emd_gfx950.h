@@ -6,8 +6,8 @@
  * or a negative status; emd_last_error() gives the message of the last failure on the
  * calling thread.  A handle is not thread-safe (same as the reference launcher).
  *
- * The library never frees caller memory.  Scratch (global knot tiles, retry list, queue
- * counters) is owned by the handle and grown on demand.
+ * The library never frees caller memory.  Scratch (global knot tiles, queue counter) is
+ * owned by the handle and grown on demand.
  */
 #ifndef EMD_GFX950_H
 #define EMD_GFX950_H
@@ -27,9 +27,8 @@ extern "C" {
  * max_extrema=1024, nbsym=2, max_iteration=1000, max_imfs=16, threads_per_block=128.
  * tile_slots: knot slots held in LDS per workgroup for both envelopes (0 = hold
  * 2*min(max_extrema, N) + 4*nbsym, the reference's "capacity = shared memory" rule,
- * kernels.py:1273-1275).  A trace whose knots outgrow the tile is redone, inside the same
- * call, by a retry launch whose tile holds any trace (LDS, or global memory when LDS is
- * too small); results are bit-identical whatever the tile. */
+ * kernels.py:1273-1275).  A sift whose knots outgrow the LDS tile keeps them in the
+ * workgroup's own global-memory tile; results are bit-identical whatever the tile. */
 typedef struct emd_config {
   int max_extrema;
   int nbsym;
