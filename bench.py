@@ -48,6 +48,18 @@ def algorithmic_bytes(n_imfs: np.ndarray, lengths: np.ndarray) -> float:
     return float(np.sum(8.0 * L + 8.0 * L * n_imfs + 4 * (2 + 3 * 16 + 4)))
 
 
+def measured_traffic():
+    """HBM bytes per launch of the persistent kernel on this workload, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (counters cannot be read inside the timed run)."""
+    import re
+    path = os.path.join(ROOT, "profiles", "emd_sift_kernels_seismic_16384x2001.md")
+    try:
+        m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", open(path).read())
+    except OSError:
+        return None
+    return float(m.group(1)) if m else None
+
+
 def _cpu_one(x):
     from oracle import pyemd_restated as O
     return O.emd(x, max_imf=15).n_imfs
@@ -75,7 +87,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--n", type=int, default=2001)
-    ap.add_argument("--mode", choices=["both", "plain", "packed"], default="both")
+    ap.add_argument("--mode", choices=["both", "plain", "packed", "copy"], default="both",
+                    help="copy: only a 1 GiB device copy, the size calibration for "
+                         "FETCH_SIZE/WRITE_SIZE counters")
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=int, default=1024)
@@ -85,6 +99,16 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every trace is "
               "independent, so N GPUs would shard the batch with no exchange.", file=sys.stderr)
         sys.exit(2)
+
+    if args.mode == "copy":
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": src.numel() * 8, "launches": 3}), flush=True)
+        return
 
     from cuda_emd import synth
 
@@ -99,28 +123,25 @@ def main():
     dx = torch.from_numpy(x).cuda()
     offsets = torch.arange(0, (B + 1) * N, N, dtype=torch.int64)
     lengths = np.full(B, N, dtype=np.int64)
-    stream = torch.cuda.current_stream()
     cfg = EMDKernelConfig.for_device()
     launcher = cfg.build()
     results = {}
+
+    launcher.set_timing(True)
 
     def timed(fn):
         for _ in range(args.warmup):
             fn()
         torch.cuda.synchronize()
-        evs = []
+        kernel_ms = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
             out = fn()
-            e1.record(stream)
-            evs.append((e0, e1))
+            # HIP events the library records on this stream around the sift launch
+            kernel_ms.append(launcher.kernel_time_ms())
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        ms = [a.elapsed_time(b) for a, b in evs]
-        return out, wall / args.steps * 1e3, float(np.mean(ms))
+        return out, wall / args.steps * 1e3, float(np.mean(kernel_ms))
 
     if args.mode in ("both", "plain"):
         out, ms_wall, ms_ev = timed(lambda: launcher.run(dx))
@@ -156,7 +177,8 @@ def main():
         "config": {"workload": "seismic traces, 16384 x N=2001, seed 1", "traces": B, "N": N,
                    "path": head, "max_imfs": cfg.max_imfs, "max_extrema": cfg.max_extrema},
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": measured_traffic() if head == "packed" and B == 16384 else None,
                      "flops_per_launch": flops,
                      "algorithmic_hbm_bytes": algorithmic_bytes(r["n_imfs"], lengths)},
     }

@@ -42,6 +42,9 @@ struct emd_handle {
   char* spill = nullptr;       // per-workgroup global knot tiles
   size_t spill_size = 0;
   unsigned* counters = nullptr;  // [0] persistent-kernel queue head
+  int timing = 0;                // record ev[0]/ev[1] around the sift launches
+  int timed = 0;                 // the last run recorded them
+  hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -203,6 +206,8 @@ int emd_create(const emd_config* cfg, emd_handle** out) {
 int emd_destroy(emd_handle* h) {
   if (!h) return EMD_OK;
   hipError_t e = hipDeviceSynchronize();
+  for (hipEvent_t ev : h->ev)
+    if (ev) (void)hipEventDestroy(ev);
   if (h->spill) (void)hipFree(h->spill);
   if (h->counters) (void)hipFree(h->counters);
   delete h;
@@ -214,12 +219,21 @@ __global__ void zero_counters_kernel(unsigned* c) {
   if (threadIdx.x < 4) c[threadIdx.x] = 0u;
 }
 
+static int mark(emd_handle* h, int which, hipStream_t st) {
+  if (!h->timing) return EMD_OK;
+  hipError_t e = hipEventRecord(h->ev[which], st);
+  if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+  if (which == 1) h->timed = 1;
+  return EMD_OK;
+}
+
 static int run_common(emd_handle* h, const double* values, const int64_t* offsets, int N,
                       int B, int max_n, const int* order, const emd_launch& L, double* imfs,
                       const int64_t* out_offsets, int* n_imfs, int* overflow, int* debug_n,
                       int* debug_extno, int* debug_nzm, int* stats, hipStream_t st) {
   if (B < 0) return fail(EMD_ERR_VALUE, "batch size must be >= 0, got %d", B);
   if (max_n < 0) return fail(EMD_ERR_VALUE, "trace length must be >= 0, got %d", max_n);
+  h->timed = 0;
   if (B == 0) return EMD_OK;
   if ((max_n > 0 && (!values || !imfs)) || !n_imfs || !overflow || !debug_n || !debug_extno ||
       !debug_nzm)
@@ -263,15 +277,18 @@ static int run_common(emd_handle* h, const double* values, const int64_t* offset
     if (rc) return rc;
     b.B = B;
     b.base = 0;
+    if ((rc = mark(h, 0, st))) return rc;
     rc = emdk::launch_sift(1, s.wg, grid, b, P, st);
     if (rc) return hip_fail((hipError_t)rc, "persistent sift launch");
-    return check_launch("persistent sift kernel");
+    if ((rc = check_launch("persistent sift kernel"))) return rc;
+    return mark(h, 1, st);
   }
   if (stride) {
     rc = ensure_spill(h, (size_t)stride * std::min(B, kMaxPlainBlocks));
     if (rc) return rc;
     b.spill = h->spill;
   }
+  if ((rc = mark(h, 0, st))) return rc;
   for (int base = 0; base < B; base += kMaxPlainBlocks) {
     const int nb = std::min(kMaxPlainBlocks, B - base);
     b.B = nb;
@@ -281,7 +298,7 @@ static int run_common(emd_handle* h, const double* values, const int64_t* offset
     rc = check_launch("sift kernel");
     if (rc) return rc;
   }
-  return EMD_OK;
+  return mark(h, 1, st);
 }
 
 int emd_run_padded(emd_handle* h, const double* signals, int B, int N, double* imfs,
@@ -362,6 +379,29 @@ int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double
     rc = check_launch("envelope kernel");
     if (rc) return rc;
   }
+  return EMD_OK;
+}
+
+int emd_set_timing(emd_handle* h, int enable) {
+  if (!h) return fail(EMD_ERR_VALUE, "null handle");
+  if (enable && !h->ev[0]) {
+    for (hipEvent_t& ev : h->ev) {
+      hipError_t e = hipEventCreate(&ev);
+      if (e != hipSuccess) return hip_fail(e, "hipEventCreate");
+    }
+  }
+  h->timing = enable ? 1 : 0;
+  h->timed = 0;
+  return EMD_OK;
+}
+
+int emd_kernel_time_ms(emd_handle* h, float* ms) {
+  if (!h || !ms) return fail(EMD_ERR_VALUE, "null argument");
+  if (!h->timed) return fail(EMD_ERR_VALUE, "no timed sift launch since timing was enabled");
+  hipError_t e = hipEventSynchronize(h->ev[1]);
+  if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+  e = hipEventElapsedTime(ms, h->ev[0], h->ev[1]);
+  if (e != hipSuccess) return hip_fail(e, "hipEventElapsedTime");
   return EMD_OK;
 }
 

@@ -49,6 +49,8 @@ PROTOTYPES = {
                                          _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "emd_max_resident": (ctypes.c_int, [_vp, ctypes.POINTER(EmdLaunchC), ctypes.c_int,
                                         _c_int_p]),
+    "emd_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "emd_kernel_time_ms": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
 }
 
 _lib = None

@@ -269,6 +269,17 @@ class EMDBatchLauncher:
             _stream()), "EMDBatchLauncher.run_envelope_only")
         return max_env, min_env, n_extrema, degenerate, mkp, mkv, nkp, nkv, n_mirrored
 
+    def set_timing(self, enable: bool = True) -> None:
+        """Record HIP events on the launch stream around the sift kernel launches of every
+        following run / run_ragged call."""
+        _lib.check(_lib.load().emd_set_timing(self._h, 1 if enable else 0), "set_timing")
+
+    def kernel_time_ms(self) -> float:
+        """Milliseconds between the events of the last timed run (waits for it)."""
+        ms = ctypes.c_float(0.0)
+        _lib.check(_lib.load().emd_kernel_time_ms(self._h, ctypes.byref(ms)), "kernel_time_ms")
+        return float(ms.value)
+
     def max_resident(self, threads: int, tile_slots: int, max_n: int, persistent=True) -> int:
         L = _lib.EmdLaunchC(threads, tile_slots, 0, 1 if persistent else 0)
         out = ctypes.c_int(0)

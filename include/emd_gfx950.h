@@ -102,6 +102,14 @@ int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double
  * EMDKernelConfig.for_device, kernels.py:1205-1271). */
 int emd_max_resident(emd_handle* h, const emd_launch* launch, int max_n, int* wgs_per_cu);
 
+/* Kernel timing: with timing enabled, every emd_run_padded / emd_run_ragged records a HIP
+ * event on the caller's stream right before its first sift launch and right after its
+ * last one; emd_kernel_time_ms waits for the second and returns the elapsed milliseconds.
+ * No reference counterpart (the reference's benchmarks time with CUDA events around the
+ * whole call, README.md:167-175). */
+int emd_set_timing(emd_handle* h, int enable);
+int emd_kernel_time_ms(emd_handle* h, float* ms);
+
 #ifdef __cplusplus
 }
 #endif
