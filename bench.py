@@ -80,6 +80,25 @@ def cpu_baseline(x: np.ndarray, n: int):
                       f"NumPy/SciPy PyEMD EMD() restatement, multiprocessing pool of {cores}"}
 
 
+def train_cost_model(launcher, args):
+    """Train the cost model (untimed) on the method's own counts for seismic traces of a
+    different seed (3) than the timed workload (1)."""
+    import torch
+    from cuda_emd import costmodel, synth
+    xt = synth.seismic_traces(args.train, args.n, seed=3)
+    v = torch.from_numpy(np.ascontiguousarray(xt.ravel())).cuda()
+    off = torch.arange(0, xt.size + 1, xt.shape[1], dtype=torch.int64).cuda()
+    t0 = time.perf_counter()
+    out = launcher.run_ragged(v, off, threads=args.threads, persistent=True)
+    f = costmodel.features(v, off)
+    cm = costmodel.CostModel(seed=0)
+    hist = cm.fit(f, costmodel.targets_from(out[7], out[2]), epochs=args.epochs, batch=1024)
+    torch.cuda.synchronize()
+    return cm, {"train_traces": int(args.train), "train_seed": 3, "epochs": args.epochs,
+                "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1],
+                "train_s": time.perf_counter() - t0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,13 +106,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--n", type=int, default=2001)
-    ap.add_argument("--mode", choices=["both", "plain", "packed", "copy"], default="both",
-                    help="copy: only a 1 GiB device copy, the size calibration for "
-                         "FETCH_SIZE/WRITE_SIZE counters")
+    ap.add_argument("--mode", choices=["both", "plain", "fifo", "packed", "copy"],
+                    default="both",
+                    help="plain: one workgroup per trace; fifo: persistent queue in batch "
+                         "order with the capacity-rule tile; packed: persistent queue ordered "
+                         "and LDS-sized by the cost model; both: all three; copy: only a 1 GiB "
+                         "device copy, the size calibration for FETCH_SIZE/WRITE_SIZE")
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tile", type=int, default=1024)
+    ap.add_argument("--tile", type=int, default=0,
+                    help="LDS knot slots of the packed path (0: sized by the cost model)")
     ap.add_argument("--threads", type=int, default=256)
+    ap.add_argument("--train", type=int, default=8192,
+                    help="seismic traces (seed 3) the cost model is trained on before timing")
+    ap.add_argument("--epochs", type=int, default=30)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every trace is "
@@ -149,15 +175,38 @@ def main():
         stats = launcher.last_stats.cpu().numpy()
         results["plain"] = dict(ms=ms_wall, kernel_ms=ms_ev, n_imfs=n_imfs, stats=stats,
                                 out=out)
-    if args.mode in ("both", "packed"):
-        def packed():
+    if args.mode in ("both", "fifo"):
+        def fifo():
             return launcher.run_ragged(dx.view(-1), offsets, threads=args.threads,
-                                       tile_slots=args.tile, persistent=True)
+                                       persistent=True)
+        out, ms_wall, ms_ev = timed(fifo)
+        results["fifo"] = dict(ms=ms_wall, kernel_ms=ms_ev, n_imfs=out[2].cpu().numpy(),
+                               stats=out[7].cpu().numpy(), out=out)
+    model_info = None
+    if args.mode in ("both", "packed"):
+        from cuda_emd import costmodel
+        cm, model_info = train_cost_model(launcher, args)
+        doff = offsets.cuda()
+        plan_tile = []
+
+        def packed():
+            # the whole learned path: features -> MLP -> queue order + tile -> sift launch
+            f = costmodel.features(dx.view(-1), doff)
+            plan = cm.plan(f, lengths, nbsym=cfg.nbsym)
+            tile = args.tile or plan.tile_slots
+            plan_tile.append(tile)
+            return launcher.run_ragged(dx.view(-1), doff, order=plan.order, threads=args.threads,
+                                       tile_slots=tile, persistent=True)
         out, ms_wall, ms_ev = timed(packed)
         results["packed"] = dict(ms=ms_wall, kernel_ms=ms_ev, n_imfs=out[2].cpu().numpy(),
                                  stats=out[7].cpu().numpy(), out=out)
+        model_info["tile_slots"] = plan_tile[-1]
+        true = np.stack([results["packed"]["stats"][:, 0], results["packed"]["n_imfs"],
+                         results["packed"]["stats"][:, 1]], axis=1).astype(np.float64)
+        model_info["mape_on_workload"] = costmodel.mape(
+            cm.predict(costmodel.features(dx.view(-1), doff)), true)
 
-    head = "packed" if "packed" in results else "plain"
+    head = next(k for k in ("packed", "fifo", "plain") if k in results)
     r = results[head]
     flops = algorithmic_flops(r["stats"], r["n_imfs"], lengths)
     achieved = flops / (r["kernel_ms"] * 1e-3) / 1e12
@@ -185,12 +234,21 @@ def main():
     if "plain" in results:
         line["plain_traces_per_s"] = B / (results["plain"]["ms"] * 1e-3)
         line["plain_ms_per_step"] = results["plain"]["ms"]
+    if "fifo" in results:
+        line["fifo_traces_per_s"] = B / (results["fifo"]["ms"] * 1e-3)
     if "packed" in results and "plain" in results:
         line["speedup_vs_plain"] = results["plain"]["ms"] / results["packed"]["ms"]
         a, b = results["plain"]["out"], results["packed"]["out"]
         line["identical_to_plain"] = bool(
             np.array_equal(a[1].cpu().numpy(), b[2].cpu().numpy())
-            and np.array_equal(a[3].cpu().numpy(), b[4].cpu().numpy()))
+            and np.array_equal(a[3].cpu().numpy(), b[4].cpu().numpy())
+            and np.array_equal(a[4].cpu().numpy(), b[5].cpu().numpy())
+            and np.array_equal(a[5].cpu().numpy(), b[6].cpu().numpy())
+            and np.array_equal(results["plain"]["stats"][:, :3], r["stats"][:, :3]))
+    if "packed" in results and "fifo" in results:
+        line["speedup_vs_fifo"] = results["fifo"]["ms"] / results["packed"]["ms"]
+    if model_info is not None:
+        line["cost_model"] = model_info
     line["sifts_per_trace"] = float(r["stats"][:, 0].mean())
     line["imfs_per_trace"] = float(r["n_imfs"].mean())
     if cpu is not None:

@@ -12,10 +12,11 @@
 #include "emd_sift.h"
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+namespace emdk {
+
 int fail(int code, const char* fmt, ...) {
   char buf[1024];
   va_list ap;
@@ -29,6 +30,13 @@ int fail(int code, const char* fmt, ...) {
 int hip_fail(hipError_t e, const char* what) {
   return fail(EMD_ERR_RUNTIME, "%s: %s", what, hipGetErrorString(e));
 }
+
+}  // namespace emdk
+
+namespace {
+
+using emdk::fail;
+using emdk::hip_fail;
 
 constexpr int kMaxPlainBlocks = 32768;  // blocks per plain sub-launch (bounds spill memory)
 

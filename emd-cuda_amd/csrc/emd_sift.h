@@ -80,4 +80,9 @@ int reset_queue(unsigned* q, hipStream_t stream);
 // Resident workgroups per CU for a kernel instance at the given dynamic LDS.
 int max_resident_wgs(int persistent, int wg, size_t dyn_lds);
 
+// Sets the thread-local message emd_last_error() returns; returns code.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+// fail() for a HIP runtime error: EMD_ERR_RUNTIME with "what: <hip message>".
+int hip_fail(hipError_t e, const char* what);
+
 }  // namespace emdk

@@ -1061,7 +1061,7 @@ __device__ __forceinline__ WgMem workgroup_mem(double* lds, char* spill, int64_t
 
 // One workgroup per trace (grid = traces of this launch).  The reference launch model.
 template <int WG>
-__global__ __launch_bounds__(WG) void emd_plain_kernel(Batch b, SiftParams P) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void emd_plain_kernel(Batch b, SiftParams P) {
   extern __shared__ double dyn_lds[];
   __shared__ SiftShared<WG> sh;
   const WgMem m = workgroup_mem(dyn_lds, b.spill, b.spill_stride, P);
@@ -1073,7 +1073,7 @@ __global__ __launch_bounds__(WG) void emd_plain_kernel(Batch b, SiftParams P) {
 // Persistent workgroups pull queue positions from an atomic counter until the b.B queue
 // entries are drained.
 template <int WG>
-__global__ __launch_bounds__(WG) void emd_persistent_kernel(Batch b, SiftParams P) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void emd_persistent_kernel(Batch b, SiftParams P) {
   extern __shared__ double dyn_lds[];
   __shared__ SiftShared<WG> sh;
   const WgMem m = workgroup_mem(dyn_lds, b.spill, b.spill_stride, P);
@@ -1091,7 +1091,7 @@ __global__ __launch_bounds__(WG) void emd_persistent_kernel(Batch b, SiftParams 
 
 // Envelope pass without sifting (diagnostic; reference envelope_batch_kernel).
 template <int WG>
-__global__ __launch_bounds__(WG) void emd_envelope_kernel(EnvBatch e, SiftParams P) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void emd_envelope_kernel(EnvBatch e, SiftParams P) {
   extern __shared__ double dyn_lds[];
   __shared__ SiftShared<WG> sh;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;

@@ -51,6 +51,13 @@ PROTOTYPES = {
                                         _c_int_p]),
     "emd_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "emd_kernel_time_ms": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    "emd_features": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp]),
+    "emd_mlp_param_count": (ctypes.c_int, []),
+    "emd_mlp_workspace_floats": (ctypes.c_int64, [ctypes.c_int]),
+    "emd_mlp_forward": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp, _vp]),
+    "emd_mlp_backward": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
+    "emd_mlp_adam": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp]),
 }
 
 _lib = None

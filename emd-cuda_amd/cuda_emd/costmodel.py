@@ -1,0 +1,203 @@
+"""costmodel.py -- the learned cost model that packs the persistent queue.
+
+A 12 -> 64 -> 64 -> 3 FP32 MLP (exact-erf GELU) predicts, per trace, the total sift
+iterations, the IMF count and the peak extrema count from cheap per-trace features.  Every
+step runs in the gfx950 library's own kernels: ``emd_features`` (features),
+``emd_mlp_forward`` (inference), ``emd_mlp_backward`` (fused forward + backward with a
+fixed-order gradient reduction) and ``emd_mlp_adam``.  It is trained on the counts the
+decomposition itself records (stats: sifts, peak extrema; n_imfs), and used only to
+
+* order the queue longest-predicted-first (LPT on predicted sifts x N), and
+* size the LDS knot tile (a high quantile of the predicted peak extrema),
+
+so decompositions are identical with or without it; FIFO order and the capacity-rule tile
+are the fallback.  No reference counterpart: the reference launches one block per signal
+in batch order (kernels.py:1214-1220) and sizes LDS from max_extrema (kernels.py:1205-1271).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+N_FEATURES = 12
+N_PARAMS = 5187
+TARGETS = ("sifts", "n_imfs", "peak_extrema")
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr() if t is not None and t.numel() > 0 else 0)
+
+
+def _stream():
+    return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def features(values, offsets):
+    """[B,12] float32 features of CSR traces (device float64 values, int64 offsets)."""
+    torch = _torch()
+    v = values if isinstance(values, torch.Tensor) else torch.from_numpy(np.asarray(values))
+    off = offsets if isinstance(offsets, torch.Tensor) else torch.from_numpy(
+        np.asarray(offsets, dtype=np.int64))
+    if v.dtype != torch.float64 or off.dtype != torch.int64:
+        raise ValueError("features need float64 values and int64 offsets")
+    v, off = v.cuda().contiguous(), off.cuda().contiguous()
+    B = off.numel() - 1
+    out = torch.empty((max(B, 0), N_FEATURES), dtype=torch.float32, device=v.device)
+    _lib.check(_lib.load().emd_features(_ptr(v), _ptr(off), B, _ptr(out), _stream()),
+               "emd_features")
+    return out
+
+
+def targets_from(stats, n_imfs):
+    """Raw targets [B,3] (sifts, n_imfs, peak extrema) from a decomposition's records."""
+    s = stats.cpu().numpy() if hasattr(stats, "cpu") else np.asarray(stats)
+    n = n_imfs.cpu().numpy() if hasattr(n_imfs, "cpu") else np.asarray(n_imfs)
+    return np.stack([s[:, 0], n, s[:, 1]], axis=1).astype(np.float64)
+
+
+def _transform(raw):
+    return np.stack([np.log1p(raw[:, 0]), raw[:, 1], np.log1p(raw[:, 2])], axis=1)
+
+
+def mape(pred, true) -> dict:
+    """Mean absolute percentage error per target over traces whose true value is > 0."""
+    out = {}
+    for k, name in enumerate(TARGETS):
+        m = true[:, k] > 0
+        out[name] = float(np.mean(np.abs(pred[m, k] - true[m, k]) / true[m, k])) if m.any() \
+            else float("nan")
+    return out
+
+
+@dataclass
+class Plan:
+    order: object            # int32 device tensor, queue order (LPT)
+    tile_slots: int          # LDS knot slots
+    predicted: np.ndarray = field(repr=False, default=None)  # [B,3] raw predictions
+
+
+class CostModel:
+    """Parameters, Adam moments and input/target standardisation live on the device."""
+
+    def __init__(self, seed: int = 0, device=None):
+        torch = _torch()
+        self.device = torch.device(device or "cuda")
+        rng = np.random.default_rng(seed)
+        parts = []
+        for fan_out, fan_in in ((64, 12), (64, 64), (3, 64)):
+            bound = 1.0 / np.sqrt(fan_in)
+            parts.append(rng.uniform(-bound, bound, fan_out * fan_in))
+            parts.append(rng.uniform(-bound, bound, fan_out))
+        self.params = torch.tensor(np.concatenate(parts), dtype=torch.float32, device=self.device)
+        assert self.params.numel() == N_PARAMS == _lib.load().emd_mlp_param_count()
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.step = 0
+        self.norm = torch.cat([torch.zeros(12), torch.ones(12)]).float().to(self.device)
+        self.tmu = np.zeros(3)
+        self.tsd = np.ones(3)
+        self.history: list[float] = []
+
+    # -- kernels -------------------------------------------------------------------------
+    def forward_std(self, feats):
+        torch = _torch()
+        f = feats.to(self.device, torch.float32).contiguous()
+        out = torch.empty((f.shape[0], 3), dtype=torch.float32, device=self.device)
+        _lib.check(_lib.load().emd_mlp_forward(_ptr(self.params), _ptr(self.norm), _ptr(f),
+                                                f.shape[0], _ptr(out), _stream()),
+                   "emd_mlp_forward")
+        return out
+
+    def loss_grad(self, feats, targets_std, target_weight):
+        """(grad [N_PARAMS+1] with the loss last) of one mini-batch."""
+        torch = _torch()
+        M = feats.shape[0]
+        ws = torch.empty((int(_lib.load().emd_mlp_workspace_floats(M)),), dtype=torch.float32,
+                         device=self.device)
+        grad = torch.empty((N_PARAMS + 1,), dtype=torch.float32, device=self.device)
+        _lib.check(_lib.load().emd_mlp_backward(
+            _ptr(self.params), _ptr(self.norm), _ptr(feats), _ptr(targets_std),
+            _ptr(target_weight), M, _ptr(ws), _ptr(grad), _stream()), "emd_mlp_backward")
+        return grad
+
+    def adam(self, grad, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.step += 1
+        _lib.check(_lib.load().emd_mlp_adam(_ptr(self.params), _ptr(self.m), _ptr(self.v),
+                                             _ptr(grad), self.step, lr, beta1, beta2, eps,
+                                             _stream()), "emd_mlp_adam")
+
+    # -- training / inference ---------------------------------------------------------------
+    def fit(self, feats, raw_targets, epochs: int = 40, batch: int = 2048, lr: float = 3e-3,
+            seed: int = 0, target_weight=(1.0, 1.0, 1.0)):
+        """Mini-batch Adam on the standardised transformed targets (log1p sifts, n_imfs,
+        log1p peak).  Returns the per-epoch mean loss."""
+        torch = _torch()
+        f = feats.to(self.device, torch.float32).contiguous()
+        M = f.shape[0]
+        mu = f.mean(0)
+        sd = f.std(0).clamp_min(1e-3)
+        self.norm = torch.cat([mu, sd]).contiguous()
+        t = _transform(np.asarray(raw_targets, dtype=np.float64))
+        self.tmu, self.tsd = t.mean(0), np.maximum(t.std(0), 1e-3)
+        ts = torch.tensor((t - self.tmu) / self.tsd, dtype=torch.float32, device=self.device)
+        tw = torch.tensor(target_weight, dtype=torch.float32, device=self.device)
+        g = torch.Generator(device=self.device)
+        g.manual_seed(seed)
+        for _ in range(epochs):
+            perm = torch.randperm(M, device=self.device, generator=g)
+            losses = []
+            for s in range(0, M, batch):
+                idx = perm[s:s + batch]
+                grad = self.loss_grad(f[idx].contiguous(), ts[idx].contiguous(), tw)
+                self.adam(grad, lr)
+                losses.append(grad[N_PARAMS:])
+            self.history.append(float(torch.cat(losses).mean()))
+        return self.history
+
+    def predict(self, feats) -> np.ndarray:
+        """Raw predictions [B,3]: sifts, n_imfs, peak extrema."""
+        y = self.forward_std(feats).double().cpu().numpy() * self.tsd + self.tmu
+        return np.stack([np.expm1(y[:, 0]), y[:, 1], np.expm1(y[:, 2])], axis=1)
+
+    def plan(self, feats, lengths, nbsym: int = 2, quantile: float = 0.995,
+             margin: float = 1.1, cap: int | None = None) -> Plan:
+        """Queue order (longest predicted sifts x N first) and LDS knot tile (both
+        envelopes of a sift at the ``quantile`` of predicted peak extrema, x margin)."""
+        torch = _torch()
+        pred = self.predict(feats)
+        L = np.asarray(lengths, dtype=np.float64)
+        cost = np.maximum(pred[:, 0], 1.0) * L
+        order = torch.from_numpy(np.argsort(-cost, kind="stable").astype(np.int32)).to(self.device)
+        peak = np.quantile(np.maximum(pred[:, 2], 0.0), quantile) if len(pred) else 0.0
+        tile = int(2 * np.ceil(peak * margin) + 4 * nbsym)
+        tile = max(tile, 16 + 4 * nbsym)
+        if cap is not None:
+            tile = min(tile, cap)
+        return Plan(order, tile, pred)
+
+    # -- persistence -------------------------------------------------------------------------
+    def state(self) -> dict:
+        return {"params": self.params.cpu().numpy(), "norm": self.norm.cpu().numpy(),
+                "tmu": self.tmu, "tsd": self.tsd}
+
+    def save(self, path: str) -> None:
+        np.savez(path, **self.state())
+
+    @classmethod
+    def load(cls, path: str, device=None) -> "CostModel":
+        torch = _torch()
+        d = np.load(path, allow_pickle=False)
+        cm = cls(device=device)
+        cm.params = torch.tensor(d["params"], dtype=torch.float32, device=cm.device)
+        cm.norm = torch.tensor(d["norm"], dtype=torch.float32, device=cm.device)
+        cm.tmu, cm.tsd = np.array(d["tmu"]), np.array(d["tsd"])
+        return cm

@@ -110,6 +110,44 @@ int emd_max_resident(emd_handle* h, const emd_launch* launch, int max_n, int* wg
 int emd_set_timing(emd_handle* h, int enable);
 int emd_kernel_time_ms(emd_handle* h, float* ms);
 
+/* ---- Cost model ----------------------------------------------------------------------
+ * A small FP32 MLP predicts, per trace, total sift iterations, IMF count and peak extrema
+ * from cheap features; the host uses the predictions only to order the persistent queue
+ * and size the LDS knot tile, so decompositions never depend on it.  None of these has a
+ * reference counterpart (the reference launches one block per signal in batch order,
+ * kernels.py:1214-1220, with LDS sized from max_extrema, kernels.py:1205-1271).
+ * All pointers are device pointers; all calls are asynchronous on `stream`. */
+#define EMD_N_FEATURES 12
+#define EMD_MLP_PARAMS 5187 /* W1[64][12] b1[64] W2[64][64] b2[64] W3[3][64] b3[3] */
+
+/* feats[B][12] (fp32) of the CSR traces (values, int64 offsets[B+1]):
+ * log N, log1p(strict extrema of the raw trace), log1p(sign changes), log(rms + 1e-12),
+ * then log(fraction + 1e-10) of the mean 256-point |rfft|^2 (non-overlapping segments,
+ * one zero-padded segment when N < 256) in bins [1,2) [2,4) [4,8) [8,16) [16,32) [32,64)
+ * [64,96) [96,129). */
+int emd_features(const double* values, const int64_t* offsets, int B, float* feats,
+                 void* stream);
+
+int emd_mlp_param_count(void);
+/* floats of workspace emd_mlp_backward needs for M samples */
+int64_t emd_mlp_workspace_floats(int M);
+
+/* out[M][3] = MLP((feats - norm[0:12]) / norm[12:24]) in standardised target units. */
+int emd_mlp_forward(const float* params, const float* norm, const float* feats, int M,
+                    float* out, void* stream);
+
+/* Fused forward + backward of L = (1/M) sum_s sum_o target_weight[o] (y_so - targets_so)^2:
+ * grad[0:EMD_MLP_PARAMS] = dL/dparams, grad[EMD_MLP_PARAMS] = L.  grad holds
+ * EMD_MLP_PARAMS + 1 floats; workspace holds emd_mlp_workspace_floats(M).  Sums over
+ * workgroups run in a fixed order, so results are reproducible. */
+int emd_mlp_backward(const float* params, const float* norm, const float* feats,
+                     const float* targets, const float* target_weight, int M, float* workspace,
+                     float* grad, void* stream);
+
+/* One Adam step (bias-corrected; step counts from 1) on params with moments m, v. */
+int emd_mlp_adam(float* params, float* m, float* v, const float* grad, int step, float lr,
+                 float beta1, float beta2, float eps, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
