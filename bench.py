@@ -99,6 +99,83 @@ def train_cost_model(launcher, args):
                 "train_s": time.perf_counter() - t0}
 
 
+def bench_ragged(args):
+    """Ragged mix: plain one-workgroup-per-trace kernel over the CSR batch, versus the
+    bucketed persistent runner in FIFO order, versus the same runner ordered and
+    LDS-sized by the cost model (trained on a seed-3 ragged mix)."""
+    import torch
+    from cuda_emd import EMDKernelConfig, costmodel, synth
+    from cuda_emd.packed import Autotuner, PackedEMD
+    B = args.batch if args.batch != 16384 else 65536
+    values, offsets, kinds = synth.ragged_mix(B, seed=2)
+    lengths = np.diff(offsets)
+    cfg = EMDKernelConfig.for_device()
+    launcher = cfg.build()
+    dv = torch.from_numpy(values).cuda()
+    doff = torch.from_numpy(offsets).cuda()
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = fn()
+        torch.cuda.synchronize()
+        return out, (time.perf_counter() - t0) / args.steps * 1e3
+
+    res = {}
+    out, ms = timed(lambda: launcher.run_ragged(dv, doff, persistent=False))
+    res["plain"] = (out, ms)
+    tuner = Autotuner(cfg, cache_path=args.tune_cache)
+    fifo = PackedEMD(cfg, None, tuner).prepare(dv, offsets)
+    res["fifo"] = timed(lambda: fifo.run())
+    tv, to, _ = synth.ragged_mix(min(args.train, 16384), seed=3)
+    tvd, tod = torch.from_numpy(tv).cuda(), torch.from_numpy(to).cuda()
+    tr = launcher.run_ragged(tvd, tod, threads=256, persistent=True)
+    cm = costmodel.CostModel(seed=0)
+    hist = cm.fit(costmodel.features(tvd, tod), costmodel.targets_from(tr[7], tr[2]),
+                  epochs=args.epochs, batch=1024)
+    learned = PackedEMD(cfg, cm, tuner).prepare(dv, offsets)
+    res["packed"] = timed(lambda: learned.run())
+    a = res["plain"][0]
+    ident = {}
+    for name in ("fifo", "packed"):
+        b = res[name][0]
+        ident[name] = bool(all(np.array_equal(a[k].cpu().numpy(), b[k].cpu().numpy())
+                               for k in range(2, 7)))
+    stats, n_imfs = res["packed"][0][7].cpu().numpy(), res["packed"][0][2].cpu().numpy()
+    flops = algorithmic_flops(stats, n_imfs, lengths)
+    ms = res["packed"][1]
+    achieved = flops / (ms * 1e-3) / 1e12
+    pred = cm.predict(costmodel.features(dv, doff))
+    true = costmodel.targets_from(res["packed"][0][7], res["packed"][0][2])
+    line = {
+        "metric": "traces/sec fp64 ragged N in [256,8192], learned-packed persistent vs plain "
+                  "kernel, 1 GPU",
+        "value": B / (ms * 1e-3), "unit": "traces/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"ragged mix, {B} traces, N log-uniform [256, 8192], seed 2",
+                   "traces": B, "mean_N": float(lengths.mean())},
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_step": flops,
+                     "note": "step time of all bucket launches (concurrent streams)"},
+        "plain_traces_per_s": B / (res["plain"][1] * 1e-3),
+        "fifo_traces_per_s": B / (res["fifo"][1] * 1e-3),
+        "speedup_vs_plain": res["plain"][1] / ms,
+        "speedup_vs_fifo": res["fifo"][1] / ms,
+        "identical_to_plain": ident,
+        "buckets": {str(b): learned.choices[b].as_dict() | {"traces": int(len(learned.members[b]))}
+                    for b in sorted(learned.choices)},
+        "cost_model": {"loss_first_epoch": hist[0], "loss_last_epoch": hist[-1],
+                       "mape_on_workload": costmodel.mape(pred, true)},
+        "overflow_traces": int((res["packed"][0][3].cpu().numpy() != 0).sum()),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +197,10 @@ def main():
     ap.add_argument("--train", type=int, default=8192,
                     help="seismic traces (seed 3) the cost model is trained on before timing")
     ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--workload", choices=["seismic", "ragged"], default="seismic",
+                    help="seismic: the headline 16384 x N=2001 traces; ragged: the ragged mix "
+                         "(N log-uniform in [256, 8192], seed 2; --batch sets B, default 65536)")
+    ap.add_argument("--tune-cache", default=None, help="autotuner JSON cache (ragged)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every trace is "
@@ -135,6 +216,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": src.numel() * 8, "launches": 3}), flush=True)
         return
+
+    if args.workload == "ragged":
+        return bench_ragged(args)
 
     from cuda_emd import synth
 

@@ -202,6 +202,13 @@ int emd_create(const emd_config* cfg, emd_handle** out) {
   }
   h->num_cus = p.multiProcessorCount;
   h->lds_max = (int)p.sharedMemPerBlock;
+  {
+    const int rc = emdk::prepare_kernels(h->lds_max);
+    if (rc != (int)hipSuccess) {
+      delete h;
+      return hip_fail((hipError_t)rc, "hipFuncSetAttribute(sift kernels)");
+    }
+  }
   e = hipMalloc(reinterpret_cast<void**>(&h->counters), 256);
   if (e != hipSuccess) {
     delete h;
@@ -235,23 +242,82 @@ static int mark(emd_handle* h, int which, hipStream_t st) {
   return EMD_OK;
 }
 
-static int run_common(emd_handle* h, const double* values, const int64_t* offsets, int N,
-                      int B, int max_n, const int* order, const emd_launch& L, double* imfs,
-                      const int64_t* out_offsets, int* n_imfs, int* overflow, int* debug_n,
-                      int* debug_extno, int* debug_nzm, int* stats, hipStream_t st) {
+// Launch plan of one run: shapes, parameters, grid and the global-tile scratch it needs.
+struct Plan {
+  Shape s;
+  emdk::SiftParams P;
+  int persistent = 0;
+  int grid = 0;
+  int64_t stride = 0;      // bytes per workgroup global tile (0: none needed)
+  size_t spill_bytes = 0;  // stride x resident workgroups
+};
+
+static int make_plan(const emd_handle* h, int B, int max_n, const emd_launch& L, Plan* pl) {
+  int rc = make_shape(h, L.threads, L.tile_slots > 0 ? L.tile_slots : h->cfg.tile_slots, max_n,
+                      &pl->s);
+  if (rc) return rc;
+  pl->P = params(h, pl->s);
+  pl->stride = pl->s.spill_slots ? emdk::tile_bytes(pl->s.spill_slots) : 0;
+  pl->persistent = L.persistent ? 1 : 0;
+  if (pl->persistent) {
+    const size_t lds = emdk::dyn_lds_bytes(pl->s.tile, pl->P.max_n);
+    const int fit = emdk::max_resident_wgs(1, pl->s.wg, lds);
+    if (fit <= 0) return fail(EMD_ERR_RUNTIME, "persistent kernel shape does not fit a CU");
+    int per_cu = L.wgs_per_cu;
+    if (per_cu <= 0 || per_cu > fit) per_cu = fit;
+    pl->grid = std::min(B, h->num_cus * per_cu);
+  } else {
+    pl->grid = std::min(B, kMaxPlainBlocks);
+  }
+  pl->spill_bytes = (size_t)pl->stride * (size_t)pl->grid;
+  return EMD_OK;
+}
+
+// Enqueue the sift launch(es) of a plan on st.  spill must hold plan.spill_bytes;
+// counters is the queue head of a persistent launch.
+static int enqueue(emd_handle* h, const Plan& pl, emdk::Batch b, int B, char* spill,
+                   unsigned* counters, bool timed, hipStream_t st) {
+  b.spill = pl.stride ? spill : nullptr;
+  b.spill_stride = pl.stride;
+  b.queue = counters;
+  int rc;
+  if (pl.persistent) {
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, st, counters);
+    if ((rc = check_launch("queue reset"))) return rc;
+    b.B = B;
+    b.base = 0;
+    if (timed && (rc = mark(h, 0, st))) return rc;
+    rc = emdk::launch_sift(1, pl.s.wg, pl.grid, b, pl.P, st);
+    if (rc) return hip_fail((hipError_t)rc, "persistent sift launch");
+    if ((rc = check_launch("persistent sift kernel"))) return rc;
+    return timed ? mark(h, 1, st) : EMD_OK;
+  }
+  if (timed && (rc = mark(h, 0, st))) return rc;
+  for (int base = 0; base < B; base += kMaxPlainBlocks) {
+    const int nb = std::min(kMaxPlainBlocks, B - base);
+    b.B = nb;
+    b.base = base;
+    rc = emdk::launch_sift(0, pl.s.wg, nb, b, pl.P, st);
+    if (rc) return hip_fail((hipError_t)rc, "sift launch");
+    if ((rc = check_launch("sift kernel"))) return rc;
+  }
+  return timed ? mark(h, 1, st) : EMD_OK;
+}
+
+static int check_io(int B, int max_n, const double* values, double* imfs, int* n_imfs,
+                    int* overflow, int* debug_n, int* debug_extno, int* debug_nzm) {
   if (B < 0) return fail(EMD_ERR_VALUE, "batch size must be >= 0, got %d", B);
   if (max_n < 0) return fail(EMD_ERR_VALUE, "trace length must be >= 0, got %d", max_n);
-  h->timed = 0;
-  if (B == 0) return EMD_OK;
-  if ((max_n > 0 && (!values || !imfs)) || !n_imfs || !overflow || !debug_n || !debug_extno ||
-      !debug_nzm)
+  if (B > 0 && ((max_n > 0 && (!values || !imfs)) || !n_imfs || !overflow || !debug_n ||
+                !debug_extno || !debug_nzm))
     return fail(EMD_ERR_VALUE, "null input or output pointer");
-  Shape s;
-  int rc = make_shape(h, L.threads, L.tile_slots > 0 ? L.tile_slots : h->cfg.tile_slots,
-                      max_n, &s);
-  if (rc) return rc;
-  const emdk::SiftParams P = params(h, s);
-  const int64_t stride = s.spill_slots ? emdk::tile_bytes(s.spill_slots) : 0;
+  return EMD_OK;
+}
+
+static emdk::Batch make_batch(const double* values, const int64_t* offsets, int N,
+                              const int* order, double* imfs, const int64_t* out_offsets,
+                              int* n_imfs, int* overflow, int* debug_n, int* debug_extno,
+                              int* debug_nzm, int* stats) {
   emdk::Batch b;
   b.values = values;
   b.offsets = offsets;
@@ -265,48 +331,23 @@ static int run_common(emd_handle* h, const double* values, const int64_t* offset
   b.dbg_e = debug_extno;
   b.dbg_z = debug_nzm;
   b.stats = stats;
-  b.spill = nullptr;
-  b.spill_stride = stride;
-  b.queue = h->counters;
-  if (L.persistent) {
-    const size_t lds = emdk::dyn_lds_bytes(s.tile, P.max_n);
-    const int fit = emdk::max_resident_wgs(1, s.wg, lds);
-    if (fit <= 0) return fail(EMD_ERR_RUNTIME, "persistent kernel shape does not fit a CU");
-    int per_cu = L.wgs_per_cu;
-    if (per_cu <= 0 || per_cu > fit) per_cu = fit;
-    const int grid = std::min(B, h->num_cus * per_cu);
-    if (stride) {
-      rc = ensure_spill(h, (size_t)stride * grid);
-      if (rc) return rc;
-      b.spill = h->spill;
-    }
-    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, st, h->counters);
-    rc = check_launch("queue reset");
-    if (rc) return rc;
-    b.B = B;
-    b.base = 0;
-    if ((rc = mark(h, 0, st))) return rc;
-    rc = emdk::launch_sift(1, s.wg, grid, b, P, st);
-    if (rc) return hip_fail((hipError_t)rc, "persistent sift launch");
-    if ((rc = check_launch("persistent sift kernel"))) return rc;
-    return mark(h, 1, st);
-  }
-  if (stride) {
-    rc = ensure_spill(h, (size_t)stride * std::min(B, kMaxPlainBlocks));
-    if (rc) return rc;
-    b.spill = h->spill;
-  }
-  if ((rc = mark(h, 0, st))) return rc;
-  for (int base = 0; base < B; base += kMaxPlainBlocks) {
-    const int nb = std::min(kMaxPlainBlocks, B - base);
-    b.B = nb;
-    b.base = base;
-    rc = emdk::launch_sift(0, s.wg, nb, b, P, st);
-    if (rc) return hip_fail((hipError_t)rc, "sift launch");
-    rc = check_launch("sift kernel");
-    if (rc) return rc;
-  }
-  return mark(h, 1, st);
+  return b;
+}
+
+static int run_common(emd_handle* h, const double* values, const int64_t* offsets, int N,
+                      int B, int max_n, const int* order, const emd_launch& L, double* imfs,
+                      const int64_t* out_offsets, int* n_imfs, int* overflow, int* debug_n,
+                      int* debug_extno, int* debug_nzm, int* stats, hipStream_t st) {
+  int rc = check_io(B, max_n, values, imfs, n_imfs, overflow, debug_n, debug_extno, debug_nzm);
+  if (rc) return rc;
+  h->timed = 0;
+  if (B == 0) return EMD_OK;
+  Plan pl;
+  if ((rc = make_plan(h, B, max_n, L, &pl))) return rc;
+  if (pl.spill_bytes && (rc = ensure_spill(h, pl.spill_bytes))) return rc;
+  const emdk::Batch b = make_batch(values, offsets, N, order, imfs, out_offsets, n_imfs,
+                                   overflow, debug_n, debug_extno, debug_nzm, stats);
+  return enqueue(h, pl, b, B, h->spill, h->counters, h->timing != 0, st);
 }
 
 int emd_run_padded(emd_handle* h, const double* signals, int B, int N, double* imfs,
@@ -388,6 +429,101 @@ int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double
     if (rc) return rc;
   }
   return EMD_OK;
+}
+
+}  // extern "C"
+
+struct emd_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  char* spill = nullptr;
+  unsigned* counters = nullptr;
+};
+
+static void graph_free(emd_graph* g) {
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->spill) (void)hipFree(g->spill);
+  if (g->counters) (void)hipFree(g->counters);
+  delete g;
+}
+
+extern "C" {
+
+int emd_chunk_graph_create(emd_handle* h, const emd_chunk* c, emd_graph** out) {
+  if (!h || !c || !out) return fail(EMD_ERR_VALUE, "null argument");
+  *out = nullptr;
+  const int max_n = c->offsets ? c->max_n : c->N;
+  if (max_n < 0 || max_n > emdk::kMaxN)
+    return fail(EMD_ERR_VALUE, "trace length %d outside [0, %d]", max_n, emdk::kMaxN);
+  int rc = check_io(c->B, max_n, c->values, c->imfs, c->n_imfs, c->overflow, c->debug_n,
+                    c->debug_extno, c->debug_nzm);
+  if (rc) return rc;
+  if (c->B < 1) return fail(EMD_ERR_VALUE, "a captured chunk needs B >= 1");
+  if (!c->row_offsets != !c->compact)
+    return fail(EMD_ERR_VALUE, "compaction needs both row_offsets and compact");
+  Plan pl;
+  if ((rc = make_plan(h, c->B, max_n, c->launch, &pl))) return rc;
+  emd_graph* g = new emd_graph();
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&g->counters), 256);
+  if (e == hipSuccess && pl.spill_bytes)
+    e = hipMalloc(reinterpret_cast<void**>(&g->spill), pl.spill_bytes);
+  if (e != hipSuccess) {
+    graph_free(g);
+    return fail(EMD_ERR_MEMORY, "hipMalloc(chunk graph scratch): %s", hipGetErrorString(e));
+  }
+  hipStream_t cs = nullptr;
+  e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    graph_free(g);
+    return hip_fail(e, "hipStreamCreateWithFlags");
+  }
+  e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    graph_free(g);
+    return hip_fail(e, "hipStreamBeginCapture");
+  }
+  const emdk::Batch b = make_batch(c->values, c->offsets, c->N, c->order, c->imfs,
+                                   c->out_offsets, c->n_imfs, c->overflow, c->debug_n,
+                                   c->debug_extno, c->debug_nzm, c->stats);
+  rc = enqueue(h, pl, b, c->B, g->spill, g->counters, false, cs);
+  if (!rc && c->compact) {
+    const int r2 = emdk::launch_compact(c->imfs, c->out_offsets, h->cfg.max_imfs, c->n_imfs,
+                                        c->offsets, c->N, c->B, c->row_offsets, c->compact, cs);
+    if (r2) rc = hip_fail((hipError_t)r2, "compaction launch");
+  }
+  e = hipStreamEndCapture(cs, &g->graph);
+  (void)hipStreamDestroy(cs);
+  if (rc) {
+    graph_free(g);
+    return rc;
+  }
+  if (e != hipSuccess) {
+    graph_free(g);
+    return hip_fail(e, "hipStreamEndCapture");
+  }
+  e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    graph_free(g);
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  *out = g;
+  return EMD_OK;
+}
+
+int emd_chunk_graph_launch(emd_graph* g, void* stream) {
+  if (!g || !g->exec) return fail(EMD_ERR_VALUE, "null graph");
+  hipError_t e = hipGraphLaunch(g->exec, (hipStream_t)stream);
+  return e == hipSuccess ? EMD_OK : hip_fail(e, "hipGraphLaunch");
+}
+
+int emd_chunk_graph_destroy(emd_graph* g) {
+  if (!g) return EMD_OK;
+  hipError_t e = hipDeviceSynchronize();
+  graph_free(g);
+  return e == hipSuccess ? EMD_OK : hip_fail(e, "hipDeviceSynchronize in graph destroy");
 }
 
 int emd_set_timing(emd_handle* h, int enable) {

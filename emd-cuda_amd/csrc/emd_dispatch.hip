@@ -39,14 +39,26 @@ static void* instance(int which, int wg) {
 static void* sift_kernel_ptr(int persistent, int wg) { return instance(persistent ? 1 : 0, wg); }
 static void* envelope_kernel_ptr(int wg) { return instance(2, wg); }
 
+int prepare_kernels(int lds_per_workgroup) {
+  // Allow each kernel the whole per-workgroup LDS once, so launches (possibly under
+  // stream capture) never change function attributes.
+  for (int wg : {64, 128, 256}) {
+    const int dyn = lds_per_workgroup - static_lds_bytes(wg);
+    for (int which = 0; which < 3; ++which) {
+      hipError_t e = hipFuncSetAttribute(instance(which, wg),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+      if (e != hipSuccess) return (int)e;
+    }
+  }
+  return (int)hipSuccess;
+}
+
 int launch_sift(int persistent, int wg, int grid, const Batch& b, const SiftParams& P,
                 hipStream_t stream) {
   void* fn = sift_kernel_ptr(persistent, wg);
   if (!fn) return (int)hipErrorInvalidValue;
   if (grid <= 0) return (int)hipSuccess;
   const size_t lds = dyn_lds_bytes(P.tile_slots, P.max_n);
-  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return (int)e;
   Batch bb = b;
   SiftParams pp = P;
   void* args[] = {&bb, &pp};
@@ -58,8 +70,6 @@ int launch_envelope(int wg, const EnvBatch& e, const SiftParams& P, hipStream_t 
   if (!fn) return (int)hipErrorInvalidValue;
   if (e.B <= 0) return (int)hipSuccess;
   const size_t lds = dyn_lds_bytes(P.tile_slots, P.max_n);
-  hipError_t r = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (r != hipSuccess) return (int)r;
   EnvBatch ee = e;
   SiftParams pp = P;
   void* args[] = {&ee, &pp};
@@ -74,9 +84,6 @@ int reset_queue(unsigned* q, hipStream_t stream) {
 int max_resident_wgs(int persistent, int wg, size_t dyn_lds) {
   void* fn = sift_kernel_ptr(persistent, wg);
   if (!fn) return 0;
-  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_lds) !=
-      hipSuccess)
-    return 0;
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, wg, dyn_lds) != hipSuccess) return 0;
   return nb;

@@ -79,6 +79,13 @@ int launch_envelope(int wg, const EnvBatch& e, const SiftParams& P, hipStream_t 
 int reset_queue(unsigned* q, hipStream_t stream);
 // Resident workgroups per CU for a kernel instance at the given dynamic LDS.
 int max_resident_wgs(int persistent, int wg, size_t dyn_lds);
+// Raise every kernel's dynamic-LDS limit to the device's per-workgroup LDS (once per handle).
+int prepare_kernels(int lds_per_workgroup);
+
+// Enqueue the compaction scan (row_offsets[B+1]) and, when compact is non-null, the row copy.
+int launch_compact(const double* imfs, const int64_t* out_offsets, int max_imfs,
+                   const int* n_imfs, const int64_t* offsets, int N, int B,
+                   int64_t* row_offsets, double* compact, hipStream_t st);
 
 // Sets the thread-local message emd_last_error() returns; returns code.
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));

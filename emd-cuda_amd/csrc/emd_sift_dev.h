@@ -864,6 +864,16 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
       io.dbg_z[k] = -1;
     }
   }
+  if (N > P.max_n || N < 0) {
+    // the launch was sized for shorter traces (or the offsets decrease): the LDS work
+    // area cannot hold this one; it is reported with overflow = -1 and no rows
+    if (tid == 0) {
+      *io.n_imfs = 0;
+      *io.overflow = -1;
+      if (io.stats) io.stats[0] = io.stats[1] = io.stats[2] = io.stats[3] = 0;
+    }
+    return;
+  }
   // The running sum of accepted IMFs (PyEMD's np.sum(IMF, axis=0), rows added in order)
   // is kept in the last output row, which no IMF occupies before the residue is written.
   double* acc = io.out + (int64_t)(P.max_imfs - 1) * N;
@@ -1031,7 +1041,8 @@ __device__ __forceinline__ SignalIO signal_io(const Batch& b, int s, int max_imf
   }
   io.S = b.values + off;
   io.N = (int)n;
-  io.out = b.imfs + (b.out_offsets ? b.out_offsets[s] : (int64_t)s * max_imfs * n);
+  // default output layout: trace s's max_imfs rows start at (input offset) x max_imfs
+  io.out = b.imfs + (b.out_offsets ? b.out_offsets[s] : off * max_imfs);
   io.n_imfs = b.n_imfs + s;
   io.overflow = b.overflow + s;
   io.dbg_n = b.dbg_n + (int64_t)s * max_imfs;

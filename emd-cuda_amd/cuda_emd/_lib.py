@@ -31,6 +31,17 @@ class EmdLaunchC(ctypes.Structure):
                 ("wgs_per_cu", ctypes.c_int), ("persistent", ctypes.c_int)]
 
 
+class EmdChunkC(ctypes.Structure):
+    _fields_ = [("values", ctypes.c_void_p), ("offsets", ctypes.c_void_p), ("N", ctypes.c_int),
+                ("B", ctypes.c_int), ("max_n", ctypes.c_int), ("order", ctypes.c_void_p),
+                ("launch", EmdLaunchC), ("imfs", ctypes.c_void_p),
+                ("out_offsets", ctypes.c_void_p), ("n_imfs", ctypes.c_void_p),
+                ("overflow", ctypes.c_void_p), ("debug_n", ctypes.c_void_p),
+                ("debug_extno", ctypes.c_void_p), ("debug_nzm", ctypes.c_void_p),
+                ("stats", ctypes.c_void_p), ("row_offsets", ctypes.c_void_p),
+                ("compact", ctypes.c_void_p)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/emd_gfx950.h
 PROTOTYPES = {
     "emd_last_error": (ctypes.c_char_p, []),
@@ -51,6 +62,13 @@ PROTOTYPES = {
                                         _c_int_p]),
     "emd_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "emd_kernel_time_ms": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    "emd_compact_offsets": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    "emd_compact": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                   _vp, _vp, _vp]),
+    "emd_chunk_graph_create": (ctypes.c_int, [_vp, ctypes.POINTER(EmdChunkC),
+                                              ctypes.POINTER(_vp)]),
+    "emd_chunk_graph_launch": (ctypes.c_int, [_vp, _vp]),
+    "emd_chunk_graph_destroy": (ctypes.c_int, [_vp]),
     "emd_features": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp]),
     "emd_mlp_param_count": (ctypes.c_int, []),
     "emd_mlp_workspace_floats": (ctypes.c_int64, [ctypes.c_int]),

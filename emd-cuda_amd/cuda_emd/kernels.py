@@ -182,12 +182,11 @@ class EMDBatchLauncher:
         c = self.config
         dev = s.device
         imfs = torch.empty((B, c.max_imfs, N), dtype=torch.float64, device=dev)
-        n_imfs = torch.zeros((B,), dtype=torch.int32, device=dev)
-        overflow = torch.zeros((B,), dtype=torch.int32, device=dev)
-        dn = torch.full((B, c.max_imfs), -1, dtype=torch.int32, device=dev)
-        de = torch.full((B, c.max_imfs), -1, dtype=torch.int32, device=dev)
-        dz = torch.full((B, c.max_imfs), -1, dtype=torch.int32, device=dev)
-        stats = torch.zeros((B, 4), dtype=torch.int32, device=dev)
+        # every element below is written by the kernel (unreached debug slots as -1)
+        i32 = dict(dtype=torch.int32, device=dev)
+        n_imfs, overflow = torch.empty((B,), **i32), torch.empty((B,), **i32)
+        dn, de, dz = (torch.empty((B, c.max_imfs), **i32) for _ in range(3))
+        stats = torch.empty((B, 4), **i32)
         _lib.check(_lib.load().emd_run_padded(
             self._h, _ptr(s), B, N, _ptr(imfs), _ptr(n_imfs), _ptr(overflow), _ptr(dn),
             _ptr(de), _ptr(dz), _ptr(stats), _stream()), "EMDBatchLauncher.run")
@@ -224,12 +223,11 @@ class EMDBatchLauncher:
             out_offsets = out_offsets.cuda().contiguous()
             total = int((out_offsets.cpu().numpy() + c.max_imfs * lengths).max()) if B else 0
         imfs = torch.empty((max(total, 1),), dtype=torch.float64, device=dev)
-        n_imfs = torch.zeros((B,), dtype=torch.int32, device=dev)
-        overflow = torch.zeros((B,), dtype=torch.int32, device=dev)
-        dn = torch.full((B, c.max_imfs), -1, dtype=torch.int32, device=dev)
-        de = torch.full((B, c.max_imfs), -1, dtype=torch.int32, device=dev)
-        dz = torch.full((B, c.max_imfs), -1, dtype=torch.int32, device=dev)
-        stats = torch.zeros((B, 4), dtype=torch.int32, device=dev)
+        # every element below is written by the kernel (unreached debug slots as -1)
+        i32 = dict(dtype=torch.int32, device=dev)
+        n_imfs, overflow = torch.empty((B,), **i32), torch.empty((B,), **i32)
+        dn, de, dz = (torch.empty((B, c.max_imfs), **i32) for _ in range(3))
+        stats = torch.empty((B, 4), **i32)
         if order is not None:
             order = (order if isinstance(order, torch.Tensor) else torch.from_numpy(
                 np.ascontiguousarray(np.asarray(order, dtype=np.int32)))).to(dev).int().contiguous()
@@ -269,6 +267,30 @@ class EMDBatchLauncher:
             _stream()), "EMDBatchLauncher.run_envelope_only")
         return max_env, min_env, n_extrema, degenerate, mkp, mkv, nkp, nkv, n_mirrored
 
+    def compact(self, imfs, n_imfs, offsets=None, N: int | None = None, out_offsets=None):
+        """Keep each trace's n_imfs rows, back to back: returns (row_offsets [B+1] int64,
+        compact fp64 [row_offsets[B]]); trace s's rows start at row_offsets[s].  ``imfs``
+        is run()'s padded output (give N) or run_ragged()'s flat output (give offsets)."""
+        torch = _torch()
+        c = self.config
+        B = int(n_imfs.numel())
+        dev = n_imfs.device
+        off = None if offsets is None else (
+            offsets if isinstance(offsets, torch.Tensor) else torch.from_numpy(
+                np.asarray(offsets, dtype=np.int64))).to(dev).contiguous()
+        if off is None and N is None:
+            N = int(imfs.shape[-1])
+        row = torch.empty((B + 1,), dtype=torch.int64, device=dev)
+        lib = _lib.load()
+        _lib.check(lib.emd_compact_offsets(_ptr(n_imfs), _ptr(off), int(N or 0), B, _ptr(row),
+                                           _stream()), "compact")
+        total = int(row[B].item()) if B else 0
+        out = torch.empty((max(total, 1),), dtype=torch.float64, device=dev)
+        _lib.check(lib.emd_compact(_ptr(imfs), _ptr(out_offsets), c.max_imfs, _ptr(n_imfs),
+                                   _ptr(off), int(N or 0), B, _ptr(row), _ptr(out), _stream()),
+                   "compact")
+        return row, out[:total]
+
     def set_timing(self, enable: bool = True) -> None:
         """Record HIP events on the launch stream around the sift kernel launches of every
         following run / run_ragged call."""
@@ -286,3 +308,96 @@ class EMDBatchLauncher:
         _lib.check(_lib.load().emd_max_resident(self._h, ctypes.byref(L), max_n,
                                                  ctypes.byref(out)), "max_resident")
         return out.value
+
+
+class ChunkGraph:
+    """One decomposition over fixed device buffers, captured once into a HIP graph.
+
+    Buffers (allocated here) hold up to ``capacity`` traces and ``values_capacity``
+    samples in CSR form (a padded chunk is CSR with uniform offsets); every trace must be
+    at most ``max_n`` long.  Refill ``values`` / ``offsets`` (and ``order``) between
+    replays -- ``load()`` does it on the current stream -- and ``launch()`` replays the
+    sift, then the compaction (``compact=True``) into ``row_offsets`` / ``compact_out``.
+    Unused trace slots are zero-length traces.  Outputs follow run_ragged's layout.
+    """
+
+    def __init__(self, launcher: "EMDBatchLauncher", capacity: int, values_capacity: int,
+                 max_n: int, threads: int | None = None, tile_slots: int = 0,
+                 wgs_per_cu: int = 0, persistent: bool = True, compact: bool = True,
+                 use_order: bool = False):
+        torch = _torch()
+        c = launcher.config
+        self.launcher = launcher
+        self.capacity, self.values_capacity, self.max_n = capacity, values_capacity, max_n
+        dev = torch.device("cuda", torch.cuda.current_device())
+        i32 = dict(dtype=torch.int32, device=dev)
+        self.values = torch.zeros((max(values_capacity, 1),), dtype=torch.float64, device=dev)
+        self.offsets = torch.zeros((capacity + 1,), dtype=torch.int64, device=dev)
+        self.order = torch.arange(capacity, **i32) if use_order else None
+        self.imfs = torch.empty((max(values_capacity * c.max_imfs, 1),), dtype=torch.float64,
+                                device=dev)
+        self.n_imfs, self.overflow = torch.empty((capacity,), **i32), torch.empty((capacity,), **i32)
+        self.debug_n, self.debug_extno, self.debug_nzm = (
+            torch.empty((capacity, c.max_imfs), **i32) for _ in range(3))
+        self.stats = torch.empty((capacity, 4), **i32)
+        self.row_offsets = torch.empty((capacity + 1,), dtype=torch.int64, device=dev) \
+            if compact else None
+        self.compact_out = torch.empty((max(values_capacity * c.max_imfs, 1),),
+                                       dtype=torch.float64, device=dev) if compact else None
+        L = _lib.EmdLaunchC(threads or c.threads_per_block, tile_slots, wgs_per_cu,
+                            1 if persistent else 0)
+        spec = _lib.EmdChunkC(_ptr(self.values).value, _ptr(self.offsets).value, 0, capacity,
+                              max_n, _ptr(self.order).value, L, _ptr(self.imfs).value, None,
+                              _ptr(self.n_imfs).value, _ptr(self.overflow).value,
+                              _ptr(self.debug_n).value, _ptr(self.debug_extno).value,
+                              _ptr(self.debug_nzm).value, _ptr(self.stats).value,
+                              _ptr(self.row_offsets).value, _ptr(self.compact_out).value)
+        torch.cuda.synchronize()
+        g = ctypes.c_void_p()
+        _lib.check(_lib.load().emd_chunk_graph_create(launcher._h, ctypes.byref(spec),
+                                                       ctypes.byref(g)), "ChunkGraph")
+        self._g = g
+        self.count = 0
+
+    def load(self, values, offsets, order=None, non_blocking: bool = True) -> None:
+        """Copy one chunk (values, int64 offsets [B+1] starting at 0, B <= capacity) into
+        the buffers on the current stream; host tensors should be pinned."""
+        torch = _torch()
+        off = offsets if isinstance(offsets, torch.Tensor) else torch.from_numpy(
+            np.asarray(offsets, dtype=np.int64))
+        B = off.numel() - 1
+        if B > self.capacity:
+            raise ValueError(f"chunk of {B} traces exceeds the capacity {self.capacity}")
+        n_vals = int(off[-1]) if off.device.type == "cpu" else int(off[-1].item())
+        if n_vals > self.values_capacity:
+            raise ValueError(f"chunk of {n_vals} samples exceeds {self.values_capacity}")
+        if off.device.type == "cpu":
+            lens = np.diff(off.numpy())
+            if B and int(lens.max()) > self.max_n:
+                raise ValueError(f"a trace of {int(lens.max())} samples exceeds max_n "
+                                 f"{self.max_n}")
+        v = values if isinstance(values, torch.Tensor) else torch.from_numpy(np.asarray(values))
+        self.values[:n_vals].copy_(v.reshape(-1)[:n_vals], non_blocking=non_blocking)
+        self.offsets[:B + 1].copy_(off, non_blocking=non_blocking)
+        self.offsets[B + 1:].fill_(n_vals)
+        if self.order is not None:
+            if order is None:
+                self.order.copy_(torch.arange(self.capacity, dtype=torch.int32))
+            else:
+                o = order if isinstance(order, torch.Tensor) else torch.from_numpy(
+                    np.asarray(order, dtype=np.int32))
+                self.order[:B].copy_(o, non_blocking=non_blocking)
+                self.order[B:].copy_(torch.arange(B, self.capacity, dtype=torch.int32))
+        self.count = B
+
+    def launch(self) -> None:
+        _lib.check(_lib.load().emd_chunk_graph_launch(self._g, _stream()), "ChunkGraph.launch")
+
+    def __del__(self):
+        g = getattr(self, "_g", None)
+        if g is not None and g.value:
+            try:
+                _lib.load().emd_chunk_graph_destroy(g)
+            except Exception:
+                pass
+            self._g = None

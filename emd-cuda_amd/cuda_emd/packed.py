@@ -1,0 +1,251 @@
+"""packed.py -- ragged batches through length buckets, an autotuner and the cost model.
+
+A ragged CSR batch is split by trace length into buckets (upper bounds 512, 1024, ...,
+16384).  Each bucket is one persistent launch over its own queue (a subset of the batch,
+addressed through the queue order), captured once into a HIP graph and replayed on its own
+stream, so the tail of one bucket overlaps the next.  Per bucket:
+
+* the cost model (when given) orders the queue longest-predicted-first (sifts x N) and
+  proposes the LDS knot tile (a high quantile of predicted peak extrema);
+* the autotuner picks the workgroup size (64/128/256), the LDS tile (the model's proposal
+  or the capacity rule) and the resident workgroups per CU by timing the candidates on a
+  sample of the bucket's traces; results are cached in a JSON file.
+
+None of these choices changes a decomposition (canonical arithmetic), which the tests
+check.  No reference counterpart: the reference needs one shared N per launch and launches
+one block per signal (kernels.py:1214-1220, 1343-1372).
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .kernels import EMDBatchLauncher, EMDKernelConfig, _ptr, _stream
+
+BUCKET_BOUNDS = (512, 1024, 2048, 4096, 8192, 16384)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def bucket_of(lengths: np.ndarray) -> np.ndarray:
+    """Bucket index of each length: the first bound >= length."""
+    return np.searchsorted(np.asarray(BUCKET_BOUNDS), np.asarray(lengths), side="left")
+
+
+@dataclass
+class LaunchChoice:
+    threads: int
+    tile_slots: int      # 0 = capacity rule
+    wgs_per_cu: int      # 0 = as many as fit
+    ms: float = float("nan")
+
+    def as_dict(self):
+        return {"threads": self.threads, "tile_slots": self.tile_slots,
+                "wgs_per_cu": self.wgs_per_cu, "ms": self.ms}
+
+
+def _device_key():
+    torch = _torch()
+    p = torch.cuda.get_device_properties(torch.cuda.current_device())
+    return f"{getattr(p, 'gcnArchName', p.name)}:{p.multi_processor_count}"
+
+
+class Autotuner:
+    """Times launch candidates per length bucket on a sample of that bucket's traces."""
+
+    def __init__(self, config: EMDKernelConfig, cache_path: str | None = None,
+                 sample: int = 1024, threads=(64, 128, 256)):
+        self.config = config
+        self.cache_path = cache_path
+        self.sample = sample
+        self.threads = tuple(threads)
+        self.cache = {}
+        if cache_path and os.path.exists(cache_path):
+            with open(cache_path) as f:
+                self.cache = json.load(f)
+
+    def _key(self, bucket: int, tile_hint: int) -> str:
+        c = self.config
+        return (f"{_device_key()}|me{c.max_extrema}|nb{c.nbsym}|it{c.max_iteration}|"
+                f"imfs{c.max_imfs}|b{BUCKET_BOUNDS[bucket]}|tile{tile_hint}")
+
+    def choose(self, launcher: EMDBatchLauncher, values, offsets_host: np.ndarray,
+               idx: np.ndarray, bucket: int, tile_hint: int = 0) -> LaunchChoice:
+        """Best launch for the traces ``idx`` of bucket ``bucket`` (tile_hint: the cost
+        model's tile, 0 = none)."""
+        key = self._key(bucket, tile_hint)
+        if key in self.cache:
+            d = self.cache[key]
+            return LaunchChoice(d["threads"], d["tile_slots"], d["wgs_per_cu"], d.get("ms", 0.0))
+        torch = _torch()
+        rng = np.random.default_rng(bucket)
+        pick = np.sort(rng.choice(idx, min(len(idx), self.sample), replace=False))
+        lens = offsets_host[pick + 1] - offsets_host[pick]
+        sub_off = np.zeros(len(pick) + 1, np.int64)
+        sub_off[1:] = np.cumsum(lens)
+        gather = np.concatenate([np.arange(offsets_host[s], offsets_host[s + 1]) for s in pick])
+        sub_vals = values[torch.from_numpy(gather).to(values.device)].contiguous()
+        sub_off_t = torch.from_numpy(sub_off).cuda()
+        tiles = [0] + ([tile_hint] if tile_hint else [])
+        launcher.set_timing(True)
+        best = None
+        for th in self.threads:
+            for tile in tiles:
+                fit = launcher.max_resident(th, tile, int(lens.max()), persistent=True)
+                for wpc in sorted({fit, max(1, fit - 1)}, reverse=True):
+                    ms = []
+                    for _ in range(2):
+                        launcher.run_ragged(sub_vals, sub_off_t, threads=th, tile_slots=tile,
+                                            wgs_per_cu=wpc, persistent=True)
+                        ms.append(launcher.kernel_time_ms())
+                    cand = LaunchChoice(th, tile, wpc, ms[-1])
+                    if best is None or cand.ms < best.ms:
+                        best = cand
+        launcher.set_timing(False)
+        self.cache[key] = best.as_dict()
+        if self.cache_path:
+            os.makedirs(os.path.dirname(os.path.abspath(self.cache_path)), exist_ok=True)
+            with open(self.cache_path, "w") as f:
+                json.dump(self.cache, f, indent=1, sort_keys=True)
+        return best
+
+
+class PackedEMD:
+    """Learned-packed persistent decomposition of one ragged CSR batch.
+
+    ``prepare`` allocates the batch's outputs, buckets the traces, runs the cost model
+    (features -> MLP) for tiles and queue order, autotunes each bucket and captures one
+    chunk graph per bucket.  ``run`` refreshes the queue order from the cost model and
+    replays the graphs on one stream per bucket (forked from and joined back to the
+    current stream).  Outputs follow run_ragged's layout (default row offsets).
+    """
+
+    def __init__(self, config: EMDKernelConfig | None = None, cost_model=None,
+                 autotuner: Autotuner | None = None, tile_quantile: float = 0.995):
+        self.config = config or EMDKernelConfig()
+        self.cost_model = cost_model
+        self.autotuner = autotuner or Autotuner(self.config)
+        self.tile_quantile = tile_quantile
+        self.launchers = {}
+        self.graphs = {}
+
+    def prepare(self, values, offsets) -> "PackedEMD":
+        torch = _torch()
+        from . import costmodel
+        c = self.config
+        self.values = values if isinstance(values, torch.Tensor) else torch.from_numpy(
+            np.ascontiguousarray(values))
+        self.values = self.values.cuda().contiguous()
+        off_host = np.asarray(offsets.cpu().numpy() if hasattr(offsets, "cpu") else offsets,
+                              dtype=np.int64)
+        self.offsets_host = off_host
+        self.offsets = torch.from_numpy(off_host).cuda()
+        B = len(off_host) - 1
+        self.B = B
+        lengths = np.diff(off_host)
+        self.lengths = lengths
+        dev = self.values.device
+        i32 = dict(dtype=torch.int32, device=dev)
+        total = int(off_host[-1])
+        self.imfs = torch.empty((max(total * c.max_imfs, 1),), dtype=torch.float64, device=dev)
+        self.n_imfs, self.overflow = torch.empty((B,), **i32), torch.empty((B,), **i32)
+        self.debug_n, self.debug_extno, self.debug_nzm = (
+            torch.empty((B, c.max_imfs), **i32) for _ in range(3))
+        self.stats = torch.empty((B, 4), **i32)
+        self.bucket = bucket_of(lengths)
+        pred = None
+        if self.cost_model is not None:
+            self.feats = costmodel.features(self.values, self.offsets)
+            pred = self.cost_model.predict(self.feats)
+        self.choices, self.order_bufs, self.members = {}, {}, {}
+        tuner_launcher = EMDBatchLauncher(None, c)
+        for b in np.unique(self.bucket):
+            idx = np.nonzero(self.bucket == b)[0]
+            bmax = int(lengths[idx].max())
+            tile_hint = 0
+            if pred is not None:
+                peak = np.quantile(np.maximum(pred[idx, 2], 0.0), self.tile_quantile)
+                tile_hint = int(2 * np.ceil(1.1 * peak) + 4 * c.nbsym)
+                tile_hint = int(np.ceil(tile_hint / 32.0) * 32)  # coarse, so the cache hits
+            ch = self.autotuner.choose(tuner_launcher, self.values, off_host, idx, int(b),
+                                       tile_hint)
+            self.choices[int(b)] = ch
+            self.members[int(b)] = idx
+            self.order_bufs[int(b)] = torch.from_numpy(
+                self._order(idx, pred).astype(np.int32)).to(dev)
+            launcher = EMDBatchLauncher(None, c)
+            self.launchers[int(b)] = launcher
+            spec = _lib.EmdChunkC(
+                _ptr(self.values).value, _ptr(self.offsets).value, 0, len(idx), bmax,
+                _ptr(self.order_bufs[int(b)]).value,
+                _lib.EmdLaunchC(ch.threads, ch.tile_slots, ch.wgs_per_cu, 1),
+                _ptr(self.imfs).value, None, _ptr(self.n_imfs).value, _ptr(self.overflow).value,
+                _ptr(self.debug_n).value, _ptr(self.debug_extno).value,
+                _ptr(self.debug_nzm).value, _ptr(self.stats).value, None, None)
+            torch.cuda.synchronize()
+            g = ctypes.c_void_p()
+            _lib.check(_lib.load().emd_chunk_graph_create(launcher._h, ctypes.byref(spec),
+                                                           ctypes.byref(g)), "PackedEMD")
+            self.graphs[int(b)] = g
+        # buckets by predicted (or length-proxy) total work, largest first
+        work = {b: float(np.sum(lengths[i] * (pred[i, 0] if pred is not None else 1.0)))
+                for b, i in self.members.items()}
+        self.launch_order = sorted(self.graphs, key=lambda b: -work[b])
+        self.streams = {b: torch.cuda.Stream() for b in self.graphs}
+        return self
+
+    def _order(self, idx, pred):
+        if pred is None:
+            return idx
+        cost = np.maximum(pred[idx, 0], 1.0) * self.lengths[idx]
+        return idx[np.argsort(-cost, kind="stable")]
+
+    def refresh_order(self) -> None:
+        """Re-derive every bucket's queue order from the cost model (the learned step of
+        each run: features -> MLP -> LPT order)."""
+        if self.cost_model is None:
+            return
+        from . import costmodel
+        pred = self.cost_model.predict(costmodel.features(self.values, self.offsets))
+        for b, idx in self.members.items():
+            self.order_bufs[b].copy_(_torch().from_numpy(self._order(idx, pred).astype(np.int32)))
+
+    def run(self, refresh: bool = True):
+        """Decompose the prepared batch; returns run_ragged's tuple (imfs, out_offsets,
+        n_imfs, overflow, debug_n, debug_extno, debug_nzm, stats)."""
+        torch = _torch()
+        if refresh:
+            self.refresh_order()
+        main = torch.cuda.current_stream()
+        start = torch.cuda.Event()
+        start.record(main)
+        for b in self.launch_order:
+            s = self.streams[b]
+            s.wait_event(start)
+            _lib.check(_lib.load().emd_chunk_graph_launch(self.graphs[b],
+                                                           ctypes.c_void_p(s.cuda_stream)),
+                       "PackedEMD.run")
+            main.wait_stream(s)
+        out_offsets = self.offsets[:-1] * self.config.max_imfs
+        return (self.imfs, out_offsets, self.n_imfs, self.overflow, self.debug_n,
+                self.debug_extno, self.debug_nzm, self.stats)
+
+    def close(self):
+        for g in self.graphs.values():
+            if g.value:
+                _lib.load().emd_chunk_graph_destroy(g)
+        self.graphs = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -77,11 +77,14 @@ int emd_run_padded(emd_handle* h, const double* signals, int B, int N, double* i
                    int* debug_nzm, int* stats, void* stream);
 
 /* Ragged traces in CSR form: trace s is values[offsets[s] .. offsets[s+1]) (int64
- * offsets, B+1 entries, max_n >= every length).  order (NULL = FIFO) gives the queue
- * order; outputs are indexed by trace, so the order never changes them.  imfs row k of
- * trace s starts at imfs + out_offsets[s] + k*len(s) (out_offsets NULL = max_imfs *
- * offsets[s]).  No reference counterpart (the reference needs one shared N); its
- * per-trace semantics are emd_batch_kernel's. */
+ * offsets).  order (NULL = FIFO over traces 0..B-1) gives the queue: the launch
+ * decomposes traces order[0..B), which may index a larger CSR batch, so one batch can be
+ * split into several launches (e.g. one per length bucket) that share its arrays.
+ * Outputs are indexed by trace, so the order never changes them.  max_n must be >= the
+ * length of every queued trace; a longer one is reported with overflow = -1 and no rows.
+ * imfs row k of trace s starts at imfs + out_offsets[s] + k*len(s) (out_offsets NULL =
+ * max_imfs * offsets[s]).  No reference counterpart (the reference needs one shared N);
+ * its per-trace semantics are emd_batch_kernel's. */
 int emd_run_ragged(emd_handle* h, const double* values, const int64_t* offsets, int B,
                    int max_n, const int* order, const emd_launch* launch, double* imfs,
                    const int64_t* out_offsets, int* n_imfs, int* overflow, int* debug_n,
@@ -109,6 +112,51 @@ int emd_max_resident(emd_handle* h, const emd_launch* launch, int max_n, int* wg
  * whole call, README.md:167-175). */
 int emd_set_timing(emd_handle* h, int enable);
 int emd_kernel_time_ms(emd_handle* h, float* ms);
+
+/* ---- Output compaction -----------------------------------------------------------------
+ * row_offsets[B+1]: exclusive scan of n_imfs[s] * len(s) (len from CSR offsets, or N when
+ * offsets is NULL); row_offsets[B] is the total.  emd_compact also copies each trace's
+ * n_imfs rows from imfs (layout as in emd_run_padded / emd_run_ragged, out_offsets NULL =
+ * default) to compact + row_offsets[s].  No reference counterpart (the reference returns
+ * the padded array, kernels.py:1353-1372). */
+int emd_compact_offsets(const int* n_imfs, const int64_t* offsets, int N, int B,
+                        int64_t* row_offsets, void* stream);
+int emd_compact(const double* imfs, const int64_t* out_offsets, int max_imfs, const int* n_imfs,
+                const int64_t* offsets, int N, int B, int64_t* row_offsets, double* compact,
+                void* stream);
+
+/* ---- Captured chunks --------------------------------------------------------------------
+ * A chunk is one decomposition over fixed device buffers (padded when offsets is NULL,
+ * else CSR), optionally followed by compaction (row_offsets and compact both non-NULL).
+ * emd_chunk_graph_create captures it once into a HIP graph with its own queue counter and
+ * global-tile scratch; each emd_chunk_graph_launch replays it on `stream` against the
+ * buffers' current contents (refill them between replays, e.g. by an H2D copy on the
+ * same stream).  Graphs of one handle may replay concurrently on different streams.
+ * No reference counterpart (the reference launches eagerly, kernels.py:1364-1371). */
+typedef struct emd_chunk {
+  const double* values;
+  const int64_t* offsets; /* NULL: padded [B,N] */
+  int N;                  /* padded length (ignored with offsets) */
+  int B;
+  int max_n;
+  const int* order;       /* queue order or NULL */
+  emd_launch launch;
+  double* imfs;
+  const int64_t* out_offsets;
+  int* n_imfs;
+  int* overflow;
+  int* debug_n;
+  int* debug_extno;
+  int* debug_nzm;
+  int* stats;             /* may be NULL */
+  int64_t* row_offsets;   /* compaction: both or neither */
+  double* compact;
+} emd_chunk;
+
+typedef struct emd_graph emd_graph;
+int emd_chunk_graph_create(emd_handle* h, const emd_chunk* chunk, emd_graph** out);
+int emd_chunk_graph_launch(emd_graph* g, void* stream);
+int emd_chunk_graph_destroy(emd_graph* g);
 
 /* ---- Cost model ----------------------------------------------------------------------
  * A small FP32 MLP predicts, per trace, total sift iterations, IMF count and peak extrema
