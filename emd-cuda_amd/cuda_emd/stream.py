@@ -49,26 +49,23 @@ class StreamingEMD:
         for k in ("debug_n", "debug_extno", "debug_nzm"):
             rec[k] = np.empty((B, c.max_imfs), np.int32)
         rec["stats"] = np.empty((B, 4), np.int32)
-        rows, row_off = [], [np.zeros(1, np.int64)]
+        chunk_rows = {}  # chunk index -> (local row offsets [count+1], rows)
         nslot = len(self.graphs)
         loaded = [torch.cuda.Event() for _ in range(nslot)]
         done = [torch.cuda.Event() for _ in range(nslot)]
-        pending = [None] * nslot  # (start, count, device record copies) per slot
-        host_side = [None] * nslot
+        pending = [None] * nslot  # (chunk index, start, count, record copies) per slot
 
         def drain(slot):
             if pending[slot] is None:
                 return
             done[slot].synchronize()
-            start, count, dev = pending[slot]
+            i, start, count, dev = pending[slot]
             for k, t in dev.items():
                 if k in rec:
                     rec[k][start:start + count] = t.numpy()[:count]
             if keep_rows:
-                g = self.graphs[slot]
-                ro = dev["row_offsets"].numpy()[:count + 1]
-                rows.append(g.compact_out[:int(ro[count])].cpu().numpy())
-                row_off.append(ro[1:] + row_off[-1][-1])
+                ro = dev["row_offsets"].numpy()[:count + 1].copy()
+                chunk_rows[i] = (ro, self.graphs[slot].compact_out[:int(ro[count])].cpu().numpy())
             pending[slot] = None
 
         for i, start in enumerate(range(0, B, self.chunk)):
@@ -89,11 +86,17 @@ class StreamingEMD:
                 done[slot].record(self.compute_stream)
             # the pinned staging buffer is reused only after this chunk's copy finished
             self.copy_stream.synchronize()
-            pending[slot] = (start, count, dev)
+            pending[slot] = (i, start, count, dev)
         for slot in range(nslot):
             drain(slot)
         torch.cuda.current_stream().wait_stream(self.compute_stream)
         if keep_rows:
-            rec["row_offsets"] = np.concatenate(row_off)
+            offs, rows, base = [np.zeros(1, np.int64)], [], 0
+            for i in sorted(chunk_rows):
+                ro, r = chunk_rows[i]
+                offs.append(ro[1:] + base)
+                rows.append(r)
+                base += int(ro[-1])
+            rec["row_offsets"] = np.concatenate(offs)
             rec["rows"] = np.concatenate(rows) if rows else np.zeros(0)
         return rec
