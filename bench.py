@@ -176,6 +176,131 @@ def bench_ragged(args):
     print(json.dumps(line), flush=True)
 
 
+def _seismic_chunks(total, seed, chunk=16384):
+    """total seismic traces (N=2001) drawn chunk by chunk from one generator."""
+    from cuda_emd import synth
+    rng = np.random.default_rng(seed)
+    parts, left = [], total
+    while left > 0:
+        parts.append(synth.seismic_traces(min(chunk, left), 2001, rng=rng))
+        left -= len(parts[-1])
+    return np.concatenate(parts)
+
+
+def bench_costmodel(args):
+    """Cost model at full size: trained on the method's own counts for --train-total seismic
+    traces (seed 3), evaluated (MAPE) on --eval-total held-out traces (seed 4): seismic plus
+    5% N(0,1) white noise at N=8192.  Also times the held-out batch through the bucketed
+    persistent runner in FIFO order and in learned order/tiles."""
+    import torch
+    from cuda_emd import EMDKernelConfig, costmodel
+    from cuda_emd.packed import Autotuner, PackedEMD
+    cfg = EMDKernelConfig.for_device()
+    launcher = cfg.build()
+    t0 = time.perf_counter()
+    xtr = _seismic_chunks(args.train_total, 3)
+    ftr, ttr = [], []
+    for s in range(0, len(xtr), 32768):
+        x = torch.from_numpy(xtr[s:s + 32768]).cuda()
+        off = torch.arange(0, x.numel() + 1, 2001, dtype=torch.int64).cuda()
+        out = launcher.run_ragged(x.view(-1), off, threads=256, tile_slots=800, persistent=True)
+        ftr.append(costmodel.features(x.view(-1), off))
+        ttr.append(costmodel.targets_from(out[7], out[2]))
+    ftr, ttr = torch.cat(ftr), np.concatenate(ttr)
+    gen_train_s = time.perf_counter() - t0
+    cm = costmodel.CostModel(seed=0)
+    t1 = time.perf_counter()
+    hist = cm.fit(ftr, ttr, epochs=args.epochs, batch=4096)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t1
+    # held-out: 95% seismic (N=2001) + 5% white N(0,1) at N=8192, shuffled
+    rng = np.random.default_rng(4)
+    n_white = int(round(0.05 * args.eval_total))
+    xs = _seismic_chunks(args.eval_total - n_white, 4)
+    traces = [xs[k] for k in range(len(xs))] + [rng.standard_normal(8192) for _ in range(n_white)]
+    kinds = np.array([0] * len(xs) + [1] * n_white)
+    perm = rng.permutation(len(traces))
+    traces = [traces[k] for k in perm]
+    kinds = kinds[perm]
+    offsets = np.zeros(len(traces) + 1, np.int64)
+    offsets[1:] = np.cumsum([len(t) for t in traces])
+    values = np.concatenate(traces)
+    dv, doff = torch.from_numpy(values).cuda(), torch.from_numpy(offsets).cuda()
+    tuner = Autotuner(cfg, cache_path=args.tune_cache)
+    res = {}
+    for name, model in (("fifo", None), ("learned", cm)):
+        p = PackedEMD(cfg, model, tuner).prepare(dv, offsets)
+        p.run()
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        out = p.run()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - ts, [out[k].cpu().numpy() for k in range(8)[2:]])
+        p.close()
+        del p, out
+        torch.cuda.empty_cache()
+    out = res["learned"][1]  # n_imfs, overflow, debug_n, debug_extno, debug_nzm, stats
+    true = costmodel.targets_from(out[5], out[0])
+    pred = cm.predict(costmodel.features(dv, doff))
+    same = all(np.array_equal(a, b) for a, b in zip(res["fifo"][1][:5], out[:5]))
+    B = len(traces)
+    line = {
+        "metric": "cost-model MAPE on held-out traces (sifts, n_imfs, peak extrema)",
+        "value": costmodel.mape(pred, true)["sifts"], "unit": "MAPE (sifts)",
+        "higher_is_better": False, "n_gpus": 1, "dtype": "f32 MLP / f64 EMD",
+        "data": "synthetic",
+        "config": {"workload": f"train {args.train_total} seismic (seed 3); eval {B} = "
+                               f"{len(xs)} seismic (seed 4) + {n_white} white N=8192"},
+        "mape_all": costmodel.mape(pred, true),
+        "mape_seismic": costmodel.mape(pred[kinds == 0], true[kinds == 0]),
+        "mape_white": costmodel.mape(pred[kinds == 1], true[kinds == 1]),
+        "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1], "epochs": args.epochs,
+        "train_decompose_and_features_s": gen_train_s, "fit_s": fit_s,
+        "eval_fifo_traces_per_s": B / res["fifo"][0],
+        "eval_learned_traces_per_s": B / res["learned"][0],
+        "learned_speedup_vs_fifo": res["fifo"][0] / res["learned"][0],
+        "identical_fifo_learned": bool(same),
+        "overflow_traces": int((out[1] != 0).sum()),
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_stress(args):
+    """262144 x N=4096 (seed 5) with max_extrema 2048, streamed from host memory through
+    captured chunks (H2D on a copy stream overlapping the persistent sift + compaction)."""
+    import torch
+    from cuda_emd import EMDKernelConfig, synth
+    from cuda_emd.stream import StreamingEMD
+    B = args.batch if args.batch != 16384 else 262144
+    x, kinds = synth.stress_mix(B, 4096, seed=5)
+    cfg = EMDKernelConfig(max_extrema=2048)
+    launcher = cfg.build()
+    s = StreamingEMD(launcher, 4096, chunk=16384, threads=256, tile_slots=args.tile)
+    s.run(x[:16384])  # warm (first replays)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rec = s.run(x)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    n = rec["n_imfs"].astype(np.int64)
+    line = {
+        "metric": "traces/sec fp64 N=4096 streamed from host (H2D included), 1 GPU",
+        "value": B / dt, "unit": "traces/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic", "seconds": dt,
+        "config": {"workload": f"stress mix {B} x N=4096, seed 5", "max_extrema": 2048,
+                   "chunk": 16384, "tile_slots": args.tile},
+        "overflow_traces": int((rec["overflow"] != 0).sum()),
+        "overflow_by_kind": {k: int(((rec["overflow"] != 0) & (kinds == i)).sum())
+                             for i, k in enumerate(("chirp", "ar1_0.99", "fm_chirp"))},
+        "imf_cap_hits": int((rec["stats"][:, 3] & 1).sum()),
+        "compacted_rows": int(n.sum()), "padded_rows": int(B * cfg.max_imfs),
+        "compacted_bytes": int(n.sum() * 4096 * 8),
+        "input_bytes": int(B * 4096 * 8),
+        "sifts_per_trace": float(rec["stats"][:, 0].mean()),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,9 +322,16 @@ def main():
     ap.add_argument("--train", type=int, default=8192,
                     help="seismic traces (seed 3) the cost model is trained on before timing")
     ap.add_argument("--epochs", type=int, default=30)
-    ap.add_argument("--workload", choices=["seismic", "ragged"], default="seismic",
+    ap.add_argument("--workload", choices=["seismic", "ragged", "costmodel", "stress"],
+                    default="seismic",
                     help="seismic: the headline 16384 x N=2001 traces; ragged: the ragged mix "
-                         "(N log-uniform in [256, 8192], seed 2; --batch sets B, default 65536)")
+                         "(N log-uniform in [256, 8192], seed 2; --batch sets B, default 65536); "
+                         "costmodel: train on 200k seismic decompositions (seed 3), evaluate "
+                         "on 262144 held-out traces (seed 4, 5%% white noise at N=8192); "
+                         "stress: 262144 x N=4096 (seed 5) streamed from host in captured "
+                         "chunks, max_extrema 2048, compacted output")
+    ap.add_argument("--train-total", type=int, default=200000)
+    ap.add_argument("--eval-total", type=int, default=262144)
     ap.add_argument("--tune-cache", default=None, help="autotuner JSON cache (ragged)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -219,6 +351,10 @@ def main():
 
     if args.workload == "ragged":
         return bench_ragged(args)
+    if args.workload == "costmodel":
+        return bench_costmodel(args)
+    if args.workload == "stress":
+        return bench_stress(args)
 
     from cuda_emd import synth
 

@@ -90,6 +90,10 @@ def load() -> ctypes.CDLL:
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it with `make -C emd-cuda_amd/csrc -j8` "
             "(or __graft_entry__.build()).  There is no CPU fallback.")
+    # torch ships its own HIP runtime under the same SONAME (libamdhip64.so.7).  Loading
+    # torch first makes this library bind to that one runtime; loading this library first
+    # would pull in the system runtime and leave torch without a usable device.
+    import torch  # noqa: F401
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in PROTOTYPES.items():
         fn = getattr(lib, name)
