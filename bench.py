@@ -176,6 +176,10 @@ def bench_ragged(args):
     print(json.dumps(line), flush=True)
 
 
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def _seismic_chunks(total, seed, chunk=16384):
     """total seismic traces (N=2001) drawn chunk by chunk from one generator."""
     from cuda_emd import synth
@@ -184,6 +188,7 @@ def _seismic_chunks(total, seed, chunk=16384):
     while left > 0:
         parts.append(synth.seismic_traces(min(chunk, left), 2001, rng=rng))
         left -= len(parts[-1])
+        _progress(f"seismic seed {seed}: {total - left}/{total}")
     return np.concatenate(parts)
 
 
@@ -208,11 +213,13 @@ def bench_costmodel(args):
         ttr.append(costmodel.targets_from(out[7], out[2]))
     ftr, ttr = torch.cat(ftr), np.concatenate(ttr)
     gen_train_s = time.perf_counter() - t0
+    _progress("training set decomposed")
     cm = costmodel.CostModel(seed=0)
     t1 = time.perf_counter()
     hist = cm.fit(ftr, ttr, epochs=args.epochs, batch=4096)
     torch.cuda.synchronize()
     fit_s = time.perf_counter() - t1
+    _progress(f"fit done in {fit_s:.1f}s")
     # held-out: 95% seismic (N=2001) + 5% white N(0,1) at N=8192, shuffled
     rng = np.random.default_rng(4)
     n_white = int(round(0.05 * args.eval_total))
@@ -236,6 +243,7 @@ def bench_costmodel(args):
         out = p.run()
         torch.cuda.synchronize()
         res[name] = (time.perf_counter() - ts, [out[k].cpu().numpy() for k in range(8)[2:]])
+        _progress(f"held-out {name}: {res[name][0]:.2f}s")
         p.close()
         del p, out
         torch.cuda.empty_cache()
@@ -272,11 +280,14 @@ def bench_stress(args):
     from cuda_emd import EMDKernelConfig, synth
     from cuda_emd.stream import StreamingEMD
     B = args.batch if args.batch != 16384 else 262144
+    _progress("generating stress mix")
     x, kinds = synth.stress_mix(B, 4096, seed=5)
+    _progress("generated")
     cfg = EMDKernelConfig(max_extrema=2048)
     launcher = cfg.build()
     s = StreamingEMD(launcher, 4096, chunk=16384, threads=256, tile_slots=args.tile)
     s.run(x[:16384])  # warm (first replays)
+    _progress("warm")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     rec = s.run(x)
