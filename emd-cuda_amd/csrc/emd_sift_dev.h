@@ -134,21 +134,54 @@ __device__ __forceinline__ int wave_sum_int(int v) {
   for (int d = 1; d < 64; d <<= 1) v += __shfl_xor(v, d, 64);
   return v;
 }
-// Perfect binary tree over the 64 lanes (lane order); every lane gets the same bits.
+// Cross-lane moves for the butterflies below (all 64 lanes active).  DPP covers the
+// levels inside 16-lane rows: quad permutations for lane^1 and lane^2, then row mirrors
+// (lane i <- 7-i, lane i <- 15-i), which pair each lane with the other half of its group;
+// once a group holds one value in every lane that is the same pairing as lane^4 / lane^8.
+// The last two levels combine the four rows' values read into scalars.
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141; // row_half_mirror
+constexpr int kDppMirror = 0x140;     // row_mirror
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(
+      (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo));
+}
+__device__ __forceinline__ double lane_d(double v, int lane) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Perfect binary tree over the 64 lanes (lane order); every lane gets the same bits:
+// level d adds the partner at lane^d, i.e. ((x0+x1)+(x2+x3))+... (fp addition commutes,
+// so both partners form bit-identical sums).
 __device__ __forceinline__ double wave_tree_sum(double v) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) v += __shfl_xor(v, d, 64);
-  return v;
+  v += dpp_d<kDppXor1>(v);
+  v += dpp_d<kDppXor2>(v);
+  v += dpp_d<kDppHalfMirror>(v);
+  v += dpp_d<kDppMirror>(v);
+  return (lane_d(v, 0) + lane_d(v, 16)) + (lane_d(v, 32) + lane_d(v, 48));
 }
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) v = fmax(v, __shfl_xor(v, d, 64));
-  return v;
+  v = fmax(v, dpp_d<kDppXor1>(v));
+  v = fmax(v, dpp_d<kDppXor2>(v));
+  v = fmax(v, dpp_d<kDppHalfMirror>(v));
+  v = fmax(v, dpp_d<kDppMirror>(v));
+  return fmax(fmax(lane_d(v, 0), lane_d(v, 16)), fmax(lane_d(v, 32), lane_d(v, 48)));
 }
 __device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) v = fmin(v, __shfl_xor(v, d, 64));
-  return v;
+  v = fmin(v, dpp_d<kDppXor1>(v));
+  v = fmin(v, dpp_d<kDppXor2>(v));
+  v = fmin(v, dpp_d<kDppHalfMirror>(v));
+  v = fmin(v, dpp_d<kDppMirror>(v));
+  return fmin(fmin(lane_d(v, 0), lane_d(v, 16)), fmin(lane_d(v, 32), lane_d(v, 48)));
 }
 template <int NW>
 __device__ __forceinline__ double waves_tree(const double* r, int stride) {
@@ -413,27 +446,49 @@ __device__ __forceinline__ void finish_side(const Tile& t, int side, int tb, int
   // Left knots sit at positions 2*lsym - idx.  A left knot equal in position to the first
   // true knot is the same sample (same value) and is dropped, which is what PyEMD's
   // delete of equal adjacent positions does; likewise on the right.
+  // Every source value is read before any knot is written, so the reads go out together
+  // instead of one LDS round trip after another (ranges hold at most nbsym + 1 entries).
+  constexpr int kR = kMaxNbsym + 1;
   const int ln = kd_len(L), rn = kd_len(R);
-  int nl = ln;
-  if (nl > 0 && 2 * lsym - kd_idx(L, t, tb, nl - 1) == t.X[tb]) --nl;
-  for (int q = 0; q < ln; ++q) {
-    const int idx = kd_idx(L, t, tb, q);
-    const double val = kd_val(L, t, tb, q);
-    bad |= side == 0 ? (val < 0.0) : (val > 0.0);
-    if (q < nl) {
-      t.X[tb - nl + q] = 2 * lsym - idx;
-      t.Y[tb - nl + q] = val;
+  int lx[kR], rx[kR];
+  double lv[kR], rv[kR];
+#pragma unroll
+  for (int q = 0; q < kR; ++q) {
+    if (q < ln) {
+      lx[q] = kd_idx(L, t, tb, q);
+      lv[q] = kd_val(L, t, tb, q);
+    }
+    if (q < rn) {
+      rx[q] = kd_idx(R, t, tb, q);
+      rv[q] = kd_val(R, t, tb, q);
     }
   }
-  const int rstart = (rn > 0 && 2 * rsym - kd_idx(R, t, tb, 0) == t.X[tb + n - 1]) ? 1 : 0;
-  // right knots go after the true ones (slots that kd_idx never reads)
-  for (int q = 0; q < rn; ++q) {
-    const int idx = kd_idx(R, t, tb, q);
-    const double val = kd_val(R, t, tb, q);
-    bad |= side == 0 ? (val < 0.0) : (val > 0.0);
-    if (q >= rstart) {
-      t.X[tb + n + q - rstart] = 2 * rsym - idx;
-      t.Y[tb + n + q - rstart] = val;
+  const int xfirst = t.X[tb], xlast = t.X[tb + n - 1];
+  int nl = ln;
+  if (nl > 0) {
+    int xl = lx[0];
+#pragma unroll
+    for (int q = 1; q < kR; ++q)
+      if (q == nl - 1) xl = lx[q];
+    if (2 * lsym - xl == xfirst) --nl;
+  }
+  const int rstart = (rn > 0 && 2 * rsym - rx[0] == xlast) ? 1 : 0;
+#pragma unroll
+  for (int q = 0; q < kR; ++q) {
+    if (q < ln) {
+      bad |= side == 0 ? (lv[q] < 0.0) : (lv[q] > 0.0);
+      if (q < nl) {
+        t.X[tb - nl + q] = 2 * lsym - lx[q];
+        t.Y[tb - nl + q] = lv[q];
+      }
+    }
+    // right knots go after the true ones (slots that kd_idx never reads)
+    if (q < rn) {
+      bad |= side == 0 ? (rv[q] < 0.0) : (rv[q] > 0.0);
+      if (q >= rstart) {
+        t.X[tb + n + q - rstart] = 2 * rsym - rx[q];
+        t.Y[tb + n + q - rstart] = rv[q];
+      }
     }
   }
   k0 = tb - nl;
@@ -552,6 +607,49 @@ __device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int
 // (M_0 and M_{n-1} eliminated), cyclic reduction over the concatenated rows, recovery of
 // the end second derivatives.  M is left in t.A indexed by knot slot.
 // =======================================================================================
+// Orders the LDS accesses of one wave: its lanes' earlier writes are visible to its
+// later reads (a wave's LDS operations complete in order).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Cyclic reduction over rows 0..mtot-1 (A sub-, Bd main, Cc super-diagonal, D right-hand
+// side): forward level s eliminates rows i-s and i+s from rows i = 2s*q + 2s - 1.
+__device__ __forceinline__ void cr_forward(const Tile& t, int mtot, int s, int first, int step) {
+  for (int q = first;; q += step) {
+    const int i = 2 * s * q + 2 * s - 1;
+    if (i >= mtot) break;
+    const int l = i - s, rr = i + s;
+    const double alpha = -t.A[i] / t.Bd[l];
+    double na = alpha * t.A[l];
+    double nb = t.Bd[i] + alpha * t.Cc[l];
+    double nd = t.D[i] + alpha * t.D[l];
+    double nc = 0.0;
+    if (rr < mtot) {
+      const double gamma = -t.Cc[i] / t.Bd[rr];
+      nc = gamma * t.Cc[rr];
+      nb = nb + gamma * t.A[rr];
+      nd = nd + gamma * t.D[rr];
+    }
+    t.A[i] = na;
+    t.Bd[i] = nb;
+    t.Cc[i] = nc;
+    t.D[i] = nd;
+  }
+}
+// Back substitution level s: rows i = 2s*q + s - 1 from the solved rows i-s and i+s.
+__device__ __forceinline__ void cr_backward(const Tile& t, int mtot, int s, int first, int step) {
+  for (int q = first;; q += step) {
+    const int i = 2 * s * q + s - 1;
+    if (i >= mtot) break;
+    const double xl = (i - s >= 0) ? t.D[i - s] : 0.0;
+    const double xr = (i + s < mtot) ? t.D[i + s] : 0.0;
+    t.D[i] = (t.D[i] - t.A[i] * xl - t.Cc[i] * xr) / t.Bd[i];
+  }
+}
+
 template <int WG>
 __device__ __forceinline__ void spline_solve(const Tile& t, const SiftShared<WG>& sh) {
   const int tid = threadIdx.x;
@@ -590,40 +688,29 @@ __device__ __forceinline__ void spline_solve(const Tile& t, const SiftShared<WG>
   }
   __syncthreads();
   if (mtot > 0) {
+    // levels with more than 64 rows use the whole workgroup; the narrow top of the
+    // reduction runs in wave 0 alone, where LDS ordering needs no workgroup barrier
     int s = 1;
-    for (; 2 * s <= mtot; s <<= 1) {
-      for (int q = tid;; q += WG) {
-        const int i = 2 * s * q + 2 * s - 1;
-        if (i >= mtot) break;
-        const int l = i - s, rr = i + s;
-        const double alpha = -t.A[i] / t.Bd[l];
-        double na = alpha * t.A[l];
-        double nb = t.Bd[i] + alpha * t.Cc[l];
-        double nd = t.D[i] + alpha * t.D[l];
-        double nc = 0.0;
-        if (rr < mtot) {
-          const double gamma = -t.Cc[i] / t.Bd[rr];
-          nc = gamma * t.Cc[rr];
-          nb = nb + gamma * t.A[rr];
-          nd = nd + gamma * t.D[rr];
-        }
-        t.A[i] = na;
-        t.Bd[i] = nb;
-        t.Cc[i] = nc;
-        t.D[i] = nd;
-      }
+    for (; 2 * s <= mtot && mtot / (2 * s) > 64; s <<= 1) {
+      cr_forward(t, mtot, s, tid, WG);
       __syncthreads();
     }
-    if (tid == 0) t.D[s - 1] = t.D[s - 1] / t.Bd[s - 1];
-    __syncthreads();
-    for (s >>= 1; s >= 1; s >>= 1) {
-      for (int q = tid;; q += WG) {
-        const int i = 2 * s * q + s - 1;
-        if (i >= mtot) break;
-        const double xl = (i - s >= 0) ? t.D[i - s] : 0.0;
-        const double xr = (i + s < mtot) ? t.D[i + s] : 0.0;
-        t.D[i] = (t.D[i] - t.A[i] * xl - t.Cc[i] * xr) / t.Bd[i];
+    const int s_split = s;
+    if (tid < 64) {
+      for (; 2 * s <= mtot; s <<= 1) {
+        cr_forward(t, mtot, s, tid, 64);
+        wave_sync();
       }
+      if (tid == 0) t.D[s - 1] = t.D[s - 1] / t.Bd[s - 1];
+      wave_sync();
+      for (s >>= 1; s >= s_split; s >>= 1) {
+        cr_backward(t, mtot, s, tid, 64);
+        wave_sync();
+      }
+    }
+    __syncthreads();
+    for (s = s_split >> 1; s >= 1; s >>= 1) {
+      cr_backward(t, mtot, s, tid, WG);
       __syncthreads();
     }
     // scatter M by knot slot into A (A's row contents are dead now)
@@ -635,13 +722,14 @@ __device__ __forceinline__ void spline_solve(const Tile& t, const SiftShared<WG>
     const int side = (tid == 0) ? 0 : (tid == 1 ? 1 : -1);
     if (side >= 0 && sh.kind[side] == kNotAKnot) {
       const int n = sh.nk[side], k0 = sh.k0[side], r0 = sh.row0[side], m = n - 2;
+      // all loads first, so they are in flight together
       const double x0 = t.X[k0], x1 = t.X[k0 + 1], x2 = t.X[k0 + 2];
-      const double h0 = x1 - x0, h1 = x2 - x1;
       const double M1 = t.D[r0], M2 = t.D[r0 + 1];
-      t.A[k0] = ((h0 + h1) * M1 - h0 * M2) / h1;
       const double xa = t.X[k0 + n - 3], xb = t.X[k0 + n - 2], xc = t.X[k0 + n - 1];
-      const double hnm3 = xb - xa, hnm2 = xc - xb;
       const double Mb = t.D[r0 + m - 1], Ma = t.D[r0 + m - 2];
+      const double h0 = x1 - x0, h1 = x2 - x1;
+      const double hnm3 = xb - xa, hnm2 = xc - xb;
+      t.A[k0] = ((h0 + h1) * M1 - h0 * M2) / h1;
       t.A[k0 + n - 1] = ((hnm3 + hnm2) * Mb - hnm2 * Ma) / hnm3;
     }
     // per-interval constants by knot slot: Bd = 1/h, Cc = h*h/6 (Bd/Cc rows are dead)
