@@ -823,45 +823,109 @@ struct SiftStats {
   double sq_new, sq_diff, sq_old, std_sum, old_max, old_min;
 };
 
+// Not-a-knot envelope of side `side` at samples i and i+1 (bits p and p+1 of 64-sample
+// block b).  The interval comes from the extrema masks of the detection that produced the
+// knots (true extrema at or before the sample plus the left knots); when mirrored knots
+// straddle the sample (only near the trace ends) it is walked to the enclosing one.
+template <int WG>
+__device__ __forceinline__ void envelope_pair(const Tile& t, const SiftShared<WG>& sh, int side,
+                                              const Work& wk, int b, int p, int i, int tb,
+                                              double& e0, double& e1) {
+  const int k0 = sh.k0[side], nk = sh.nk[side];
+  const uint64_t m = side == 0 ? wk.mmax[b] : wk.mmin[b];
+  const int pre = side == 0 ? wk.pmax[b] : wk.pmin[b];
+  const uint64_t le_mask = p == 63 ? ~uint64_t(0) : ((uint64_t(2) << p) - 1);
+  const int le0 = pre + __popcll(m & le_mask);
+  const int le1 = le0 + (int)((m >> (p + 1)) & 1);  // p is even, so p + 1 <= 63
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int ii = i + q;
+    int j = (tb - k0) + (q ? le1 : le0) - 1;
+    j = max(0, min(j, nk - 2));
+    int xa = t.X[k0 + j], xb = t.X[k0 + j + 1];
+    if (!((xa <= ii || j == 0) && (xb > ii || j == nk - 2))) {
+      while (j < nk - 2 && t.X[k0 + j + 1] <= ii) ++j;
+      while (j > 0 && t.X[k0 + j] > ii) --j;
+      xa = t.X[k0 + j];
+      xb = t.X[k0 + j + 1];
+    }
+    const int jj = k0 + j;
+    const double h1 = t.Bd[jj];
+    const double A = double(xb - ii) * h1;
+    const double B = double(ii - xa) * h1;
+    const double e = A * t.Y[jj] + B * t.Y[jj + 1] +
+                     ((A * A * A - A) * t.A[jj] + (B * B * B - B) * t.A[jj + 1]) * t.Cc[jj];
+    if (q == 0) e0 = e; else e1 = e;
+  }
+}
+
 // One sift: evaluate both envelopes at every sample, subtract the mean, accumulate the
-// PyEMD stop statistics.  The IMF in LDS is updated in place.
+// PyEMD stop statistics.  The IMF in LDS is updated in place.  Each lane takes two
+// adjacent samples of a 128-sample block; the pair is added first, then the 64-lane
+// tree, then the tree over blocks -- the same perfect binary tree over next_pow2(N)
+// samples (zero-padded) as the 64-sample blocks of the other reductions.
 template <int WG>
 __device__ __forceinline__ SiftStats envelope_update(const Work& wk, const Tile& t,
                                                      SiftShared<WG>& sh, int N,
                                                      const DetectResult& D, int nb) {
   constexpr int NW = WG / 64;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int nblk = ceil_div(N, 64);
+  const int nblk2 = ceil_div(N, 128);
   const int tb0 = nb, tb1 = 3 * nb + D.nm;
+  const int kind0 = uni(sh.kind[0]), kind1 = uni(sh.kind[1]);
   double omax = -INFINITY, omin = INFINITY;
-  for (int blk = w; blk < nblk; blk += NW) {
-    const int i = blk * 64 + lane;
+  for (int c = w; c < nblk2; c += NW) {
+    const int i = c * 128 + 2 * lane;
     double ln = 0.0, ld = 0.0, lo = 0.0, ls = 0.0;
     if (i < N) {
-      const double emax = envelope_at(t, sh, 0, wk, blk, lane, i, tb0);
-      const double emin = envelope_at(t, sh, 1, wk, blk, lane, i, tb1);
-      const double mean = 0.5 * (emax + emin);
-      const double old = wk.imf[i];
-      const double nw = old - mean;
-      const double diff = nw - old;
-      const double r = diff / nw;
-      ln = nw * nw;
-      ld = diff * diff;
-      lo = old * old;
-      ls = r * r;
-      omax = fmax(omax, old);
-      omin = fmin(omin, old);
-      wk.imf[i] = nw;
+      const int b = 2 * c + (lane >> 5), p = (2 * lane) & 63;
+      double emax0, emax1, emin0, emin1;
+      if (kind0 == kNotAKnot) {
+        envelope_pair(t, sh, 0, wk, b, p, i, tb0, emax0, emax1);
+      } else {
+        emax0 = eval_small(t, sh.k0[0], kind0, sh.p3[0], i);
+        emax1 = eval_small(t, sh.k0[0], kind0, sh.p3[0], i + 1);
+      }
+      if (kind1 == kNotAKnot) {
+        envelope_pair(t, sh, 1, wk, b, p, i, tb1, emin0, emin1);
+      } else {
+        emin0 = eval_small(t, sh.k0[1], kind1, sh.p3[1], i);
+        emin1 = eval_small(t, sh.k0[1], kind1, sh.p3[1], i + 1);
+      }
+      // i + 1 < 64 * nbcap always (i is even and < N), so the pair read stays in the work area
+      const double old0 = wk.imf[i], old1 = wk.imf[i + 1];
+      const double nw0 = old0 - 0.5 * (emax0 + emin0);
+      const double d0 = nw0 - old0;
+      const double r0 = d0 / nw0;
+      ln = nw0 * nw0;
+      ld = d0 * d0;
+      lo = old0 * old0;
+      ls = r0 * r0;
+      omax = fmax(omax, old0);
+      omin = fmin(omin, old0);
+      wk.imf[i] = nw0;
+      if (i + 1 < N) {
+        const double nw1 = old1 - 0.5 * (emax1 + emin1);
+        const double d1 = nw1 - old1;
+        const double r1 = d1 / nw1;
+        ln = ln + nw1 * nw1;
+        ld = ld + d1 * d1;
+        lo = lo + old1 * old1;
+        ls = ls + r1 * r1;
+        omax = fmax(omax, old1);
+        omin = fmin(omin, old1);
+        wk.imf[i + 1] = nw1;
+      }
     }
     ln = wave_tree_sum(ln);
     ld = wave_tree_sum(ld);
     lo = wave_tree_sum(lo);
     ls = wave_tree_sum(ls);
     if (lane == 0) {
-      wk.bsum[blk] = ln;
-      wk.bsum[wk.nbcap + blk] = ld;
-      wk.bsum[2 * wk.nbcap + blk] = lo;
-      wk.bsum[3 * wk.nbcap + blk] = ls;
+      wk.bsum[c] = ln;
+      wk.bsum[wk.nbcap + c] = ld;
+      wk.bsum[2 * wk.nbcap + c] = lo;
+      wk.bsum[3 * wk.nbcap + c] = ls;
     }
   }
   omax = wave_max(omax);
@@ -872,10 +936,10 @@ __device__ __forceinline__ SiftStats envelope_update(const Work& wk, const Tile&
   }
   __syncthreads();
   SiftStats st;
-  st.sq_new = blocks_tree(wk.bsum, nblk, lane);
-  st.sq_diff = blocks_tree(wk.bsum + wk.nbcap, nblk, lane);
-  st.sq_old = blocks_tree(wk.bsum + 2 * wk.nbcap, nblk, lane);
-  st.std_sum = blocks_tree(wk.bsum + 3 * wk.nbcap, nblk, lane);
+  st.sq_new = blocks_tree(wk.bsum, nblk2, lane);
+  st.sq_diff = blocks_tree(wk.bsum + wk.nbcap, nblk2, lane);
+  st.sq_old = blocks_tree(wk.bsum + 2 * wk.nbcap, nblk2, lane);
+  st.std_sum = blocks_tree(wk.bsum + 3 * wk.nbcap, nblk2, lane);
   double gmx = sh.red[0][4], gmn = sh.red[0][5];
 #pragma unroll
   for (int q = 1; q < NW; ++q) {
