@@ -159,6 +159,14 @@ __device__ __forceinline__ double lane_d(double v, int lane) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// Orders the LDS accesses of one wave: its lanes' earlier writes are visible to its
+// later reads (a wave's LDS operations complete in order).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Perfect binary tree over the 64 lanes (lane order); every lane gets the same bits:
 // level d adds the partner at lane^d, i.e. ((x0+x1)+(x2+x3))+... (fp addition commutes,
 // so both partners form bit-identical sums).
@@ -430,24 +438,65 @@ __device__ __forceinline__ int kd_slot(const KDesc& d, int q) {
   }
   return d.rev ? d.to - 1 - q : d.from + q;
 }
-__device__ __forceinline__ int kd_idx(const KDesc& d, const Tile& t, int tb, int q) {
-  const int s = kd_slot(d, q);
-  return s < 0 ? d.si : t.X[tb + s];
+// The first and last nbsym+1 true knots of each side, which are all the mirror reads:
+// lanes 0..4F-1 of the mirroring wave each load one (position, value) in a single LDS
+// round trip, and the wave-uniform mirror logic fetches them with readlane.
+struct KnotEnds {
+  int xi;      // this lane's cached knot position
+  double yv;   // this lane's cached knot value
+  int F;       // knots cached at each end of a side
+  int n[2];    // true knots per side
+};
+
+__device__ __forceinline__ KnotEnds load_ends(const Tile& t, int tb0, int tb1, int nm, int nn,
+                                              int F, int lane) {
+  KnotEnds c;
+  c.F = F;
+  c.n[0] = nm;
+  c.n[1] = nn;
+  c.xi = 0;
+  c.yv = 0.0;
+  if (lane < 4 * F) {
+    const int side = lane / (2 * F), half = (lane / F) & 1, k = lane % F;
+    const int n = side ? nn : nm;
+    const int slot = max(0, min(half ? n - F + k : k, n - 1));
+    const int tb = side ? tb1 : tb0;
+    c.xi = t.X[tb + slot];
+    c.yv = t.Y[tb + slot];
+  }
+  return c;
 }
-__device__ __forceinline__ double kd_val(const KDesc& d, const Tile& t, int tb, int q) {
-  const int s = kd_slot(d, q);
-  return s < 0 ? d.sv : t.Y[tb + s];
+// lane holding true knot `slot` of `side` (slot within the first or last F)
+__device__ __forceinline__ int ends_lane(const KnotEnds& c, int side, int slot) {
+  const int k = slot < c.F ? slot : c.F + slot - (c.n[side] - c.F);
+  return side * 2 * c.F + k;
+}
+__device__ __forceinline__ int ends_x(const KnotEnds& c, int side, int slot) {
+  return __builtin_amdgcn_readlane(c.xi, ends_lane(c, side, slot));
+}
+__device__ __forceinline__ double ends_y(const KnotEnds& c, int side, int slot) {
+  return lane_d(c.yv, ends_lane(c, side, slot));
 }
 
-__device__ __forceinline__ void finish_side(const Tile& t, int side, int tb, int n,
-                                            const KDesc& L, int lsym, const KDesc& R,
+__device__ __forceinline__ int kd_idx(const KDesc& d, const KnotEnds& c, int side, int q) {
+  const int s = kd_slot(d, q);
+  return s < 0 ? d.si : ends_x(c, side, s);
+}
+__device__ __forceinline__ double kd_val(const KDesc& d, const KnotEnds& c, int side, int q) {
+  const int s = kd_slot(d, q);
+  return s < 0 ? d.sv : ends_y(c, side, s);
+}
+
+__device__ __forceinline__ void finish_side(const Tile& t, const KnotEnds& c, int side, int tb,
+                                            int n, const KDesc& L, int lsym, const KDesc& R,
                                             int rsym, int& k0, int& nk, int& kind, int& bad,
                                             double* p3) {
+  const bool writer = (threadIdx.x & 63) == 0;
   // Left knots sit at positions 2*lsym - idx.  A left knot equal in position to the first
   // true knot is the same sample (same value) and is dropped, which is what PyEMD's
   // delete of equal adjacent positions does; likewise on the right.
-  // Every source value is read before any knot is written, so the reads go out together
-  // instead of one LDS round trip after another (ranges hold at most nbsym + 1 entries).
+  // Runs wave-uniformly; sources come from the cached knot ends (ranges hold at most
+  // nbsym + 1 entries), and lane 0 writes the mirrored knots.
   constexpr int kR = kMaxNbsym + 1;
   const int ln = kd_len(L), rn = kd_len(R);
   int lx[kR], rx[kR];
@@ -455,15 +504,15 @@ __device__ __forceinline__ void finish_side(const Tile& t, int side, int tb, int
 #pragma unroll
   for (int q = 0; q < kR; ++q) {
     if (q < ln) {
-      lx[q] = kd_idx(L, t, tb, q);
-      lv[q] = kd_val(L, t, tb, q);
+      lx[q] = kd_idx(L, c, side, q);
+      lv[q] = kd_val(L, c, side, q);
     }
     if (q < rn) {
-      rx[q] = kd_idx(R, t, tb, q);
-      rv[q] = kd_val(R, t, tb, q);
+      rx[q] = kd_idx(R, c, side, q);
+      rv[q] = kd_val(R, c, side, q);
     }
   }
-  const int xfirst = t.X[tb], xlast = t.X[tb + n - 1];
+  const int xfirst = ends_x(c, side, 0), xlast = ends_x(c, side, n - 1);
   int nl = ln;
   if (nl > 0) {
     int xl = lx[0];
@@ -477,7 +526,7 @@ __device__ __forceinline__ void finish_side(const Tile& t, int side, int tb, int
   for (int q = 0; q < kR; ++q) {
     if (q < ln) {
       bad |= side == 0 ? (lv[q] < 0.0) : (lv[q] > 0.0);
-      if (q < nl) {
+      if (q < nl && writer) {
         t.X[tb - nl + q] = 2 * lsym - lx[q];
         t.Y[tb - nl + q] = lv[q];
       }
@@ -485,7 +534,7 @@ __device__ __forceinline__ void finish_side(const Tile& t, int side, int tb, int
     // right knots go after the true ones (slots that kd_idx never reads)
     if (q < rn) {
       bad |= side == 0 ? (rv[q] < 0.0) : (rv[q] > 0.0);
-      if (q >= rstart) {
+      if (q >= rstart && writer) {
         t.X[tb + n + q - rstart] = 2 * rsym - rx[q];
         t.Y[tb + n + q - rstart] = rv[q];
       }
@@ -496,6 +545,7 @@ __device__ __forceinline__ void finish_side(const Tile& t, int side, int tb, int
   kind = nk >= 4 ? kNotAKnot : nk == 3 ? kThreePoint : nk == 2 ? kLinear : kConst;
   if (kind == kThreePoint) {
     // PyEMD cubic_spline_3pts: natural cubic spline in slope form, solved in closed form
+    wave_sync();  // lane 0's knot writes are visible to the whole wave
     const double x0 = t.X[k0], x1 = t.X[k0 + 1], x2 = t.X[k0 + 2];
     const double y0 = t.Y[k0], y1 = t.Y[k0 + 1], y2 = t.Y[k0 + 2];
     const double h1 = x1 - x0, h2 = x2 - x1;
@@ -513,16 +563,18 @@ __device__ __forceinline__ void finish_side(const Tile& t, int side, int tb, int
   }
 }
 
-// Boundary mirroring (one lane).  PyEMD _prepare_points_simple with T = arange(N); the
-// reference's mirror_one_side (kernels.py:360-558) follows the same branch table.
+// Boundary mirroring, run wave-uniformly by one wave (its results and knot writes are the
+// same in every lane; lane 0 writes).  PyEMD _prepare_points_simple with T = arange(N);
+// the reference's mirror_one_side (kernels.py:360-558) follows the same branch table.
 __device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int nb, double S0,
                                        double SN, int* k0, int* nk, int* kind, int& bad,
                                        double (*p3)[4]) {
   const int tb0 = nb, tb1 = 3 * nb + nm;
-  const int Xm0 = t.X[tb0], XmL = t.X[tb0 + nm - 1];
-  const int Xn0 = t.X[tb1], XnL = t.X[tb1 + nn - 1];
-  const double Ym0 = t.Y[tb0], YmL = t.Y[tb0 + nm - 1];
-  const double Yn0 = t.Y[tb1], YnL = t.Y[tb1 + nn - 1];
+  const KnotEnds c = load_ends(t, tb0, tb1, nm, nn, nb + 1, threadIdx.x & 63);
+  const int Xm0 = ends_x(c, 0, 0), XmL = ends_x(c, 0, nm - 1);
+  const int Xn0 = ends_x(c, 1, 0), XnL = ends_x(c, 1, nn - 1);
+  const double Ym0 = ends_y(c, 0, 0), YmL = ends_y(c, 0, nm - 1);
+  const double Yn0 = ends_y(c, 1, 0), YnL = ends_y(c, 1, nn - 1);
   KDesc lmax, lmin, rmax, rmin;
   int lsym, rsym;
   if (Xm0 < Xn0) {
@@ -587,19 +639,19 @@ __device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int
   if (kd_len(rmax) == 0) rmax = kd(0, nm, 0);
 
   // mirrored points must reach the signal edges; otherwise mirror about the edge
-  if (2 * lsym - kd_idx(lmin, t, tb1, 0) > 0 || 2 * lsym - kd_idx(lmax, t, tb0, 0) > 0) {
+  if (2 * lsym - kd_idx(lmin, c, 1, 0) > 0 || 2 * lsym - kd_idx(lmax, c, 0, 0) > 0) {
     if (lsym == Xm0) lmax = kd(0, min(nm, nb), 1);
     else lmin = kd(0, min(nn, nb), 1);
     lsym = 0;
   }
-  if (2 * rsym - kd_idx(rmin, t, tb1, kd_len(rmin) - 1) < N - 1 ||
-      2 * rsym - kd_idx(rmax, t, tb0, kd_len(rmax) - 1) < N - 1) {
+  if (2 * rsym - kd_idx(rmin, c, 1, kd_len(rmin) - 1) < N - 1 ||
+      2 * rsym - kd_idx(rmax, c, 0, kd_len(rmax) - 1) < N - 1) {
     if (rsym == XmL) rmax = kd(max(nm - nb, 0), nm, 1);
     else rmin = kd(max(nn - nb, 0), nn, 1);
     rsym = N - 1;
   }
-  finish_side(t, 0, tb0, nm, lmax, lsym, rmax, rsym, k0[0], nk[0], kind[0], bad, p3[0]);
-  finish_side(t, 1, tb1, nn, lmin, lsym, rmin, rsym, k0[1], nk[1], kind[1], bad, p3[1]);
+  finish_side(t, c, 0, tb0, nm, lmax, lsym, rmax, rsym, k0[0], nk[0], kind[0], bad, p3[0]);
+  finish_side(t, c, 1, tb1, nn, lmin, lsym, rmin, rsym, k0[1], nk[1], kind[1], bad, p3[1]);
 }
 
 // =======================================================================================
@@ -607,14 +659,6 @@ __device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int
 // (M_0 and M_{n-1} eliminated), cyclic reduction over the concatenated rows, recovery of
 // the end second derivatives.  M is left in t.A indexed by knot slot.
 // =======================================================================================
-// Orders the LDS accesses of one wave: its lanes' earlier writes are visible to its
-// later reads (a wave's LDS operations complete in order).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Cyclic reduction over rows 0..mtot-1 (A sub-, Bd main, Cc super-diagonal, D right-hand
 // side): forward level s eliminates rows i-s and i+s from rows i = 2s*q + 2s - 1.
 __device__ __forceinline__ void cr_forward(const Tile& t, int mtot, int s, int first, int step) {
@@ -968,16 +1012,25 @@ template <int WG>
 __device__ __forceinline__ void build_envelopes(const Tile& t, SiftShared<WG>& sh,
                                                 const DetectResult& D, int N,
                                                 const SiftParams& P, const Work& wk) {
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < 64) {
     int bad = sh.bad;
-    mirror(t, N, D.nm, D.nn, P.nbsym, wk.imf[0], wk.imf[N - 1], sh.k0, sh.nk, sh.kind, bad,
-           sh.p3);
-    sh.bad = bad;
-    const int m0 = sh.kind[0] == kNotAKnot ? sh.nk[0] - 2 : 0;
-    const int m1 = sh.kind[1] == kNotAKnot ? sh.nk[1] - 2 : 0;
-    sh.row0[0] = 0;
-    sh.row0[1] = m0;
-    sh.mtot = m0 + m1;
+    int k0[2], nk[2], kind[2];
+    double p3[2][4];
+    mirror(t, N, D.nm, D.nn, P.nbsym, wk.imf[0], wk.imf[N - 1], k0, nk, kind, bad, p3);
+    if (threadIdx.x == 0) {
+      sh.bad = bad;
+      for (int q = 0; q < 2; ++q) {
+        sh.k0[q] = k0[q];
+        sh.nk[q] = nk[q];
+        sh.kind[q] = kind[q];
+        for (int r = 0; r < 4; ++r) sh.p3[q][r] = p3[q][r];
+      }
+      const int m0 = kind[0] == kNotAKnot ? nk[0] - 2 : 0;
+      const int m1 = kind[1] == kNotAKnot ? nk[1] - 2 : 0;
+      sh.row0[0] = 0;
+      sh.row0[1] = m0;
+      sh.mtot = m0 + m1;
+    }
   }
   __syncthreads();
   spline_solve<WG>(t, sh);
