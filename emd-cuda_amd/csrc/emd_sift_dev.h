@@ -655,108 +655,160 @@ __device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int
 }
 
 // =======================================================================================
-// Joint not-a-knot spline solve for both envelopes: setup of the reduced (n-2) systems
-// (M_0 and M_{n-1} eliminated), cyclic reduction over the concatenated rows, recovery of
-// the end second derivatives.  M is left in t.A indexed by knot slot.
+// Not-a-knot spline solves, one wave per envelope.  Each side's system has m = knots - 2
+// rows (M_0 and M_{n-1} eliminated) and is solved by a partition method: lane l owns a
+// contiguous chunk [s, e] of rows; a Thomas sweep over the chunk's interior expresses
+// every interior unknown as x = u - v*x_s - w*x_e; the chunk end rows then form a
+// tridiagonal system in (x_s, x_e) over all chunks, which an odd-even step reduces to
+// one unknown per lane and parallel cyclic reduction across lanes solves with shuffles.
+// The systems are diagonally dominant, so no pivoting is needed.  The solution is left
+// in t.D by row; the end second derivatives are then recovered and M is stored in t.A
+// by knot slot.
 // =======================================================================================
-// Cyclic reduction over rows 0..mtot-1 (A sub-, Bd main, Cc super-diagonal, D right-hand
-// side): forward level s eliminates rows i-s and i+s from rows i = 2s*q + 2s - 1.
-__device__ __forceinline__ void cr_forward(const Tile& t, int mtot, int s, int first, int step) {
-  for (int q = first;; q += step) {
-    const int i = 2 * s * q + 2 * s - 1;
-    if (i >= mtot) break;
-    const int l = i - s, rr = i + s;
-    const double alpha = -t.A[i] / t.Bd[l];
-    double na = alpha * t.A[l];
-    double nb = t.Bd[i] + alpha * t.Cc[l];
-    double nd = t.D[i] + alpha * t.D[l];
-    double nc = 0.0;
-    if (rr < mtot) {
-      const double gamma = -t.Cc[i] / t.Bd[rr];
-      nc = gamma * t.Cc[rr];
-      nb = nb + gamma * t.A[rr];
-      nd = nd + gamma * t.D[rr];
-    }
-    t.A[i] = na;
-    t.Bd[i] = nb;
-    t.Cc[i] = nc;
-    t.D[i] = nd;
+// Row j of the system of a side whose knots start at slot k0: a x_{j-1} + b x_j + c x_{j+1} = d.
+__device__ __forceinline__ void nak_row(const Tile& t, int k0, int m, int j, double& a, double& b,
+                                        double& c, double& d) {
+  const int i = k0 + j + 1;
+  const double xa = t.X[i - 1], xb = t.X[i], xc = t.X[i + 1];
+  const double ya = t.Y[i - 1], yb = t.Y[i], yc = t.Y[i + 1];
+  const double hl = xb - xa, hr = xc - xb;
+  const double rhs = 6.0 * ((yc - yb) / hr - (yb - ya) / hl);
+  if (j == 0) {
+    a = 0.0;
+    b = hl + 2.0 * hr;
+    c = hr - hl;
+    d = hr * rhs / (hl + hr);
+  } else if (j == m - 1) {
+    a = hl - hr;
+    b = 2.0 * hl + hr;
+    c = 0.0;
+    d = hl * rhs / (hl + hr);
+  } else {
+    a = hl;
+    b = 2.0 * (hl + hr);
+    c = hr;
+    d = rhs;
   }
 }
-// Back substitution level s: rows i = 2s*q + s - 1 from the solved rows i-s and i+s.
-__device__ __forceinline__ void cr_backward(const Tile& t, int mtot, int s, int first, int step) {
-  for (int q = first;; q += step) {
-    const int i = 2 * s * q + s - 1;
-    if (i >= mtot) break;
-    const double xl = (i - s >= 0) ? t.D[i - s] : 0.0;
-    const double xr = (i + s < mtot) ? t.D[i + s] : 0.0;
-    t.D[i] = (t.D[i] - t.A[i] * xl - t.Cc[i] * xr) / t.Bd[i];
+
+// Solve one side's system (rows r0 .. r0+m-1 of the tile's row arrays) with the calling
+// wave; every lane of the wave must call it.
+__device__ __forceinline__ void spline_system_wave(const Tile& t, int k0, int m, int r0,
+                                                   int lane) {
+  const int K = min(64, m / 2);  // chunks of >= 2 rows
+  const int base = m / K, rem = m % K;
+  const bool act = lane < K;
+  const int s = act ? lane * base + min(lane, rem) : 0;
+  const int e = act ? s + base + (lane < rem ? 1 : 0) - 1 : 0;
+  // chunk end rows after eliminating the interior (identity rows on idle lanes)
+  double rsa = 0.0, rsb = 1.0, rsc = 0.0, rsd = 0.0;
+  double rea = 0.0, reb = 1.0, rec = 0.0, red = 0.0;
+  if (act) {
+    double a, b, c, d;
+    double us = 0.0, vs = 0.0, ws = 0.0, ue = 0.0, ve = 0.0, we = 0.0;
+    const bool interior = e - s >= 2;
+    if (interior) {
+      // forward sweep: Bd = g (modified super-diagonal), D = u, A = v, Cc = w
+      double gp = 0.0, up = 0.0, vp = 0.0, wp = 0.0;
+      for (int j = s + 1; j < e; ++j) {
+        nak_row(t, k0, m, j, a, b, c, d);
+        const bool first = j == s + 1, last = j == e - 1;
+        const double rb = 1.0 / (first ? b : b - a * gp);
+        const double g = last ? 0.0 : c * rb;
+        const double u = (first ? d : d - a * up) * rb;
+        const double v = (first ? a : -(a * vp)) * rb;
+        const double w = ((last ? c : 0.0) - (first ? 0.0 : a * wp)) * rb;
+        t.Bd[r0 + j] = g;
+        t.D[r0 + j] = u;
+        t.A[r0 + j] = v;
+        t.Cc[r0 + j] = w;
+        gp = g;
+        up = u;
+        vp = v;
+        wp = w;
+      }
+      ue = up;
+      ve = vp;
+      we = wp;
+      // backward sweep
+      for (int j = e - 2; j > s; --j) {
+        const double g = t.Bd[r0 + j];
+        up = t.D[r0 + j] - g * up;
+        vp = t.A[r0 + j] - g * vp;
+        wp = t.Cc[r0 + j] - g * wp;
+        t.D[r0 + j] = up;
+        t.A[r0 + j] = vp;
+        t.Cc[r0 + j] = wp;
+      }
+      us = up;
+      vs = vp;
+      ws = wp;
+    }
+    nak_row(t, k0, m, s, a, b, c, d);
+    rsa = a;
+    rsb = interior ? b - c * vs : b;
+    rsc = interior ? -(c * ws) : c;
+    rsd = interior ? d - c * us : d;
+    nak_row(t, k0, m, e, a, b, c, d);
+    rea = interior ? -(a * ve) : a;
+    reb = interior ? b - a * we : b;
+    rec = c;
+    red = interior ? d - a * ue : d;
+  }
+  // odd-even step: eliminate x_s of this chunk and of the next from the e-row
+  const double nsa = __shfl_down(rsa, 1), nsb = __shfl_down(rsb, 1);
+  const double nsc = __shfl_down(rsc, 1), nsd = __shfl_down(rsd, 1);
+  const bool has_next = lane + 1 < K;
+  const double k1 = rea / rsb;
+  const double k2 = has_next ? rec / nsb : 0.0;
+  double A = -(k1 * rsa);
+  double B = reb - k1 * rsc - (has_next ? k2 * nsa : 0.0);
+  double C = has_next ? -(k2 * nsc) : 0.0;
+  double D = red - k1 * rsd - (has_next ? k2 * nsd : 0.0);
+  // parallel cyclic reduction over the chunk e-rows
+  for (int dd = 1; dd < K; dd <<= 1) {
+    const double Au = __shfl_up(A, dd), Bu = __shfl_up(B, dd);
+    const double Cu = __shfl_up(C, dd), Du = __shfl_up(D, dd);
+    const double Ad = __shfl_down(A, dd), Bdn = __shfl_down(B, dd);
+    const double Cd = __shfl_down(C, dd), Dd = __shfl_down(D, dd);
+    const bool hu = lane >= dd, hd = lane + dd < 64;
+    const double q1 = hu ? A / Bu : 0.0;
+    const double q2 = hd ? C / Bdn : 0.0;
+    const double nA = hu ? -(q1 * Au) : 0.0;
+    const double nC = hd ? -(q2 * Cd) : 0.0;
+    const double nB = B - (hu ? q1 * Cu : 0.0) - (hd ? q2 * Ad : 0.0);
+    const double nD = D - (hu ? q1 * Du : 0.0) - (hd ? q2 * Dd : 0.0);
+    A = nA;
+    B = nB;
+    C = nC;
+    D = nD;
+  }
+  const double xe = D / B;
+  const double xprev = __shfl_up(xe, 1);
+  if (act) {
+    const double xs = (rsd - (lane > 0 ? rsa * xprev : 0.0) - rsc * xe) / rsb;
+    t.D[r0 + s] = xs;
+    t.D[r0 + e] = xe;
+    for (int j = s + 1; j < e; ++j)
+      t.D[r0 + j] = t.D[r0 + j] - t.A[r0 + j] * xs - t.Cc[r0 + j] * xe;
   }
 }
 
 template <int WG>
 __device__ __forceinline__ void spline_solve(const Tile& t, const SiftShared<WG>& sh) {
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int mtot = uni(sh.mtot);
   const int row1 = uni(sh.row0[1]);
-  for (int r = tid; r < mtot; r += WG) {
-    const int side = (r < row1) ? 0 : 1;
-    const int k = r - sh.row0[side];
-    const int m = sh.nk[side] - 2;
-    const int i = sh.k0[side] + k + 1;
-    const double xa = t.X[i - 1], xb = t.X[i], xc = t.X[i + 1];
-    const double ya = t.Y[i - 1], yb = t.Y[i], yc = t.Y[i + 1];
-    const double hl = xb - xa, hr = xc - xb;
-    const double rhs = 6.0 * ((yc - yb) / hr - (yb - ya) / hl);
-    double a, b, cc, d;
-    if (k == 0) {
-      a = 0.0;
-      b = hl + 2.0 * hr;
-      cc = hr - hl;
-      d = hr * rhs / (hl + hr);
-    } else if (k == m - 1) {
-      a = hl - hr;
-      b = 2.0 * hl + hr;
-      cc = 0.0;
-      d = hl * rhs / (hl + hr);
-    } else {
-      a = hl;
-      b = 2.0 * (hl + hr);
-      cc = hr;
-      d = rhs;
-    }
-    t.A[r] = a;
-    t.Bd[r] = b;
-    t.Cc[r] = cc;
-    t.D[r] = d;
-  }
-  __syncthreads();
   if (mtot > 0) {
-    // levels with more than 64 rows use the whole workgroup; the narrow top of the
-    // reduction runs in wave 0 alone, where LDS ordering needs no workgroup barrier
-    int s = 1;
-    for (; 2 * s <= mtot && mtot / (2 * s) > 64; s <<= 1) {
-      cr_forward(t, mtot, s, tid, WG);
-      __syncthreads();
-    }
-    const int s_split = s;
-    if (tid < 64) {
-      for (; 2 * s <= mtot; s <<= 1) {
-        cr_forward(t, mtot, s, tid, 64);
-        wave_sync();
-      }
-      if (tid == 0) t.D[s - 1] = t.D[s - 1] / t.Bd[s - 1];
-      wave_sync();
-      for (s >>= 1; s >= s_split; s >>= 1) {
-        cr_backward(t, mtot, s, tid, 64);
-        wave_sync();
-      }
+    // wave 0 solves the maxima envelope and wave 1 the minima (wave 0 both when WG = 64);
+    // their row ranges are disjoint
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int owner = WG >= 128 ? side : 0;
+      if (wv == owner && uni(sh.kind[side]) == kNotAKnot)
+        spline_system_wave(t, uni(sh.k0[side]), uni(sh.nk[side]) - 2, uni(sh.row0[side]), lane);
     }
     __syncthreads();
-    for (s = s_split >> 1; s >= 1; s >>= 1) {
-      cr_backward(t, mtot, s, tid, WG);
-      __syncthreads();
-    }
     // scatter M by knot slot into A (A's row contents are dead now)
     for (int r = tid; r < mtot; r += WG) {
       const int side = (r < row1) ? 0 : 1;
