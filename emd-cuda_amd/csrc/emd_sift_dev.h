@@ -409,9 +409,8 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
 struct KDesc {
   int from, to;  // slice [from, to) of the side's true extrema
   int rev;       // 1: elements ind[to-1], ..., ind[from]
-  int sp;        // 0: no edge sample, 1: edge sample first, 2: edge sample last
-  int si;        // edge sample index
-  double sv;     // edge sample value
+  int sp;        // 0: no edge sample; 1: sample N-1 first (right lists); 2: sample 0 last
+                 // (left lists)
 };
 
 __device__ __forceinline__ KDesc kd(int from, int to, int rev) {
@@ -420,8 +419,6 @@ __device__ __forceinline__ KDesc kd(int from, int to, int rev) {
   d.to = to;
   d.rev = rev;
   d.sp = 0;
-  d.si = 0;
-  d.sv = 0.0;
   return d;
 }
 __device__ __forceinline__ int kd_len(const KDesc& d) {
@@ -446,11 +443,16 @@ struct KnotEnds {
   double yv;   // this lane's cached knot value
   int F;       // knots cached at each end of a side
   int n[2];    // true knots per side
+  int last;    // N - 1: the right edge sample
+  double S0, SN;  // edge sample values
 };
 
 __device__ __forceinline__ KnotEnds load_ends(const Tile& t, int tb0, int tb1, int nm, int nn,
-                                              int F, int lane) {
+                                              int F, int lane, int N, double S0, double SN) {
   KnotEnds c;
+  c.last = N - 1;
+  c.S0 = S0;
+  c.SN = SN;
   c.F = F;
   c.n[0] = nm;
   c.n[1] = nn;
@@ -468,7 +470,8 @@ __device__ __forceinline__ KnotEnds load_ends(const Tile& t, int tb0, int tb1, i
 }
 // lane holding true knot `slot` of `side` (slot within the first or last F)
 __device__ __forceinline__ int ends_lane(const KnotEnds& c, int side, int slot) {
-  const int k = slot < c.F ? slot : c.F + slot - (c.n[side] - c.F);
+  const int n = side ? c.n[1] : c.n[0];
+  const int k = slot < c.F ? slot : c.F + slot - (n - c.F);
   return side * 2 * c.F + k;
 }
 __device__ __forceinline__ int ends_x(const KnotEnds& c, int side, int slot) {
@@ -480,74 +483,81 @@ __device__ __forceinline__ double ends_y(const KnotEnds& c, int side, int slot) 
 
 __device__ __forceinline__ int kd_idx(const KDesc& d, const KnotEnds& c, int side, int q) {
   const int s = kd_slot(d, q);
-  return s < 0 ? d.si : ends_x(c, side, s);
+  return s < 0 ? (d.sp == 1 ? c.last : 0) : ends_x(c, side, s);
 }
 __device__ __forceinline__ double kd_val(const KDesc& d, const KnotEnds& c, int side, int q) {
   const int s = kd_slot(d, q);
-  return s < 0 ? d.sv : ends_y(c, side, s);
+  return s < 0 ? (d.sp == 1 ? c.SN : c.S0) : ends_y(c, side, s);
 }
 
-__device__ __forceinline__ void finish_side(const Tile& t, const KnotEnds& c, int side, int tb,
-                                            int n, const KDesc& L, int lsym, const KDesc& R,
-                                            int rsym, int& k0, int& nk, int& kind, int& bad,
-                                            double* p3) {
-  const bool writer = (threadIdx.x & 63) == 0;
-  // Left knots sit at positions 2*lsym - idx.  A left knot equal in position to the first
-  // true knot is the same sample (same value) and is dropped, which is what PyEMD's
-  // delete of equal adjacent positions does; likewise on the right.
-  // Runs wave-uniformly; sources come from the cached knot ends (ranges hold at most
-  // nbsym + 1 entries), and lane 0 writes the mirrored knots.
-  constexpr int kR = kMaxNbsym + 1;
+// Knot counts of one side after PyEMD's removal of duplicate adjacent positions.
+__device__ __forceinline__ void side_counts(const KnotEnds& c, int side, int n, int tb,
+                                            const KDesc& L, int lsym, const KDesc& R, int rsym,
+                                            int& nl, int& rs, int& k0, int& nk, int& kind) {
   const int ln = kd_len(L), rn = kd_len(R);
-  int lx[kR], rx[kR];
-  double lv[kR], rv[kR];
-#pragma unroll
-  for (int q = 0; q < kR; ++q) {
-    if (q < ln) {
-      lx[q] = kd_idx(L, c, side, q);
-      lv[q] = kd_val(L, c, side, q);
-    }
-    if (q < rn) {
-      rx[q] = kd_idx(R, c, side, q);
-      rv[q] = kd_val(R, c, side, q);
-    }
-  }
-  const int xfirst = ends_x(c, side, 0), xlast = ends_x(c, side, n - 1);
-  int nl = ln;
-  if (nl > 0) {
-    int xl = lx[0];
-#pragma unroll
-    for (int q = 1; q < kR; ++q)
-      if (q == nl - 1) xl = lx[q];
-    if (2 * lsym - xl == xfirst) --nl;
-  }
-  const int rstart = (rn > 0 && 2 * rsym - rx[0] == xlast) ? 1 : 0;
-#pragma unroll
-  for (int q = 0; q < kR; ++q) {
-    if (q < ln) {
-      bad |= side == 0 ? (lv[q] < 0.0) : (lv[q] > 0.0);
-      if (q < nl && writer) {
-        t.X[tb - nl + q] = 2 * lsym - lx[q];
-        t.Y[tb - nl + q] = lv[q];
-      }
-    }
-    // right knots go after the true ones (slots that kd_idx never reads)
-    if (q < rn) {
-      bad |= side == 0 ? (rv[q] < 0.0) : (rv[q] > 0.0);
-      if (q >= rstart && writer) {
-        t.X[tb + n + q - rstart] = 2 * rsym - rx[q];
-        t.Y[tb + n + q - rstart] = rv[q];
-      }
-    }
-  }
+  nl = ln;
+  if (ln > 0 && 2 * lsym - kd_idx(L, c, side, ln - 1) == ends_x(c, side, 0)) --nl;
+  rs = (rn > 0 && 2 * rsym - kd_idx(R, c, side, 0) == ends_x(c, side, n - 1)) ? 1 : 0;
   k0 = tb - nl;
-  nk = nl + n + (rn - rstart);
+  nk = nl + n + (rn - rs);
   kind = nk >= 4 ? kNotAKnot : nk == 3 ? kThreePoint : nk == 2 ? kLinear : kConst;
-  if (kind == kThreePoint) {
+}
+
+// The four mirror lists (left/right x maxima/minima) are written in one parallel step:
+// lane q takes element q % R of list q / R (R = nbsym + 1 entries at most), so every
+// mirrored knot is gathered, checked and written at once.  Left knots sit at positions
+// 2*lsym - idx; a left knot equal in position to the first true knot is the same sample
+// (same value) and is dropped, which is what PyEMD's delete of equal adjacent positions
+// does; likewise on the right.  Right knots go after the true ones.
+__device__ __forceinline__ void finish_lists(const Tile& t, const KnotEnds& c, int tb0, int tb1,
+                                             int nm, int nn, const KDesc& lmax, const KDesc& lmin,
+                                             const KDesc& rmax, const KDesc& rmin, int lsym,
+                                             int rsym, int* k0, int* nk, int* kind, int& bad,
+                                             double (*p3)[4]) {
+  constexpr int kR = kMaxNbsym + 1;
+  const int lane = threadIdx.x & 63;
+  int nl[2], rs[2];
+  side_counts(c, 0, nm, tb0, lmax, lsym, rmax, rsym, nl[0], rs[0], k0[0], nk[0], kind[0]);
+  side_counts(c, 1, nn, tb1, lmin, lsym, rmin, rsym, nl[1], rs[1], k0[1], nk[1], kind[1]);
+  // list of this lane: 0 left maxima, 1 right maxima, 2 left minima, 3 right minima
+  const int list = lane / kR, e = lane - list * kR;
+  const int side = list >> 1;
+  const bool right = (list & 1) != 0;
+  KDesc d;  // field-wise select (a per-lane choice of a whole struct would go to scratch)
+#define EMD_PICK(f) (list == 0 ? +lmax.f : list == 1 ? +rmax.f : list == 2 ? +lmin.f : +rmin.f)
+  d.from = EMD_PICK(from);
+  d.to = EMD_PICK(to);
+  d.rev = EMD_PICK(rev);
+  d.sp = EMD_PICK(sp);
+#undef EMD_PICK
+  const bool act = list < 4 && e < kd_len(d);
+  const int slot = act ? kd_slot(d, e) : 0;
+  const int src = ends_lane(c, side, max(slot, 0));
+  const int gx = __shfl(c.xi, src);
+  const double gy = __shfl(c.yv, src);
+  const int idx = slot < 0 ? (d.sp == 1 ? c.last : 0) : gx;
+  const double val = slot < 0 ? (d.sp == 1 ? c.SN : c.S0) : gy;
+  const bool viol = act && (side == 0 ? (val < 0.0) : (val > 0.0));
+  bad |= (__ballot(viol) != 0ull) ? 1 : 0;
+  if (act) {
+    const int n = side ? nn : nm, tb = side ? tb1 : tb0;
+    const int nls = side ? nl[1] : nl[0], rss = side ? rs[1] : rs[0];
+    if (!right && e < nls) {
+      t.X[tb - nls + e] = 2 * lsym - idx;
+      t.Y[tb - nls + e] = val;
+    } else if (right && e >= rss) {
+      t.X[tb + n + e - rss] = 2 * rsym - idx;
+      t.Y[tb + n + e - rss] = val;
+    }
+  }
+  if (kind[0] == kThreePoint || kind[1] == kThreePoint) wave_sync();
+#pragma unroll
+  for (int sd = 0; sd < 2; ++sd) {
+    if (kind[sd] != kThreePoint) continue;
     // PyEMD cubic_spline_3pts: natural cubic spline in slope form, solved in closed form
-    wave_sync();  // lane 0's knot writes are visible to the whole wave
-    const double x0 = t.X[k0], x1 = t.X[k0 + 1], x2 = t.X[k0 + 2];
-    const double y0 = t.Y[k0], y1 = t.Y[k0 + 1], y2 = t.Y[k0 + 2];
+    const int q0 = k0[sd];
+    const double x0 = t.X[q0], x1 = t.X[q0 + 1], x2 = t.X[q0 + 2];
+    const double y0 = t.Y[q0], y1 = t.Y[q0 + 1], y2 = t.Y[q0 + 2];
     const double h1 = x1 - x0, h2 = x2 - x1;
     const double dy1 = y1 - y0, dy2 = y2 - y1;
     const double i1 = 1.0 / h1, i2 = 1.0 / h2;
@@ -556,10 +566,10 @@ __device__ __forceinline__ void finish_side(const Tile& t, const KnotEnds& c, in
     const double k1 = (v1 + v3) / (3.0 * (i1 + i2));
     const double kk0 = (v1 - i1 * k1) / (2.0 * i1);
     const double kk2 = (v3 - i2 * k1) / (2.0 * i2);
-    p3[0] = kk0 * h1 - dy1;   // a1
-    p3[1] = -k1 * h1 + dy1;   // b1
-    p3[2] = k1 * h2 - dy2;    // a2
-    p3[3] = -kk2 * h2 + dy2;  // b2
+    p3[sd][0] = kk0 * h1 - dy1;   // a1
+    p3[sd][1] = -k1 * h1 + dy1;   // b1
+    p3[sd][2] = k1 * h2 - dy2;    // a2
+    p3[sd][3] = -kk2 * h2 + dy2;  // b2
   }
 }
 
@@ -570,7 +580,7 @@ __device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int
                                        double SN, int* k0, int* nk, int* kind, int& bad,
                                        double (*p3)[4]) {
   const int tb0 = nb, tb1 = 3 * nb + nm;
-  const KnotEnds c = load_ends(t, tb0, tb1, nm, nn, nb + 1, threadIdx.x & 63);
+  const KnotEnds c = load_ends(t, tb0, tb1, nm, nn, nb + 1, threadIdx.x & 63, N, S0, SN);
   const int Xm0 = ends_x(c, 0, 0), XmL = ends_x(c, 0, nm - 1);
   const int Xn0 = ends_x(c, 1, 0), XnL = ends_x(c, 1, nn - 1);
   const double Ym0 = ends_y(c, 0, 0), YmL = ends_y(c, 0, nm - 1);
@@ -578,7 +588,7 @@ __device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int
   KDesc lmax, lmin, rmax, rmin;
   int lsym, rsym;
   if (Xm0 < Xn0) {
-    if (S0 > Yn0) {
+    if (uni(S0 > Yn0 ? 1 : 0)) {
       lmax = kd(1, min(nm, nb + 1), 1);
       lmin = kd(0, min(nn, nb), 1);
       lsym = Xm0;
@@ -586,39 +596,33 @@ __device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int
       lmax = kd(0, min(nm, nb), 1);
       lmin = kd(0, min(nn, nb - 1), 1);
       lmin.sp = 2;
-      lmin.si = 0;
-      lmin.sv = S0;
       lsym = 0;
     }
   } else {
-    if (S0 < Ym0) {
+    if (uni(S0 < Ym0 ? 1 : 0)) {
       lmax = kd(0, min(nm, nb), 1);
       lmin = kd(1, min(nn, nb + 1), 1);
       lsym = Xn0;
     } else {
       lmax = kd(0, min(nm, nb - 1), 1);
       lmax.sp = 2;
-      lmax.si = 0;
-      lmax.sv = S0;
       lmin = kd(0, min(nn, nb), 1);
       lsym = 0;
     }
   }
   if (XmL < XnL) {
-    if (SN < YmL) {
+    if (uni(SN < YmL ? 1 : 0)) {
       rmax = kd(max(nm - nb, 0), nm, 1);
       rmin = kd(max(nn - nb - 1, 0), nn - 1, 1);
       rsym = XnL;
     } else {
       rmax = kd(max(nm - nb + 1, 0), nm, 1);
       rmax.sp = 1;
-      rmax.si = N - 1;
-      rmax.sv = SN;
       rmin = kd(max(nn - nb, 0), nn, 1);
       rsym = N - 1;
     }
   } else {
-    if (SN > YnL) {
+    if (uni(SN > YnL ? 1 : 0)) {
       rmax = kd(max(nm - nb - 1, 0), nm - 1, 1);
       rmin = kd(max(nn - nb, 0), nn, 1);
       rsym = XmL;
@@ -626,8 +630,6 @@ __device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int
       rmax = kd(max(nm - nb, 0), nm, 1);
       rmin = kd(max(nn - nb + 1, 0), nn, 1);
       rmin.sp = 1;
-      rmin.si = N - 1;
-      rmin.sv = SN;
       rsym = N - 1;
     }
   }
@@ -650,8 +652,8 @@ __device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int
     else rmin = kd(max(nn - nb, 0), nn, 1);
     rsym = N - 1;
   }
-  finish_side(t, c, 0, tb0, nm, lmax, lsym, rmax, rsym, k0[0], nk[0], kind[0], bad, p3[0]);
-  finish_side(t, c, 1, tb1, nn, lmin, lsym, rmin, rsym, k0[1], nk[1], kind[1], bad, p3[1]);
+  finish_lists(t, c, tb0, tb1, nm, nn, lmax, lmin, rmax, rmin, lsym, rsym, k0, nk, kind, bad,
+               p3);
 }
 
 // =======================================================================================
