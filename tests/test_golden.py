@@ -1,0 +1,65 @@
+"""Golden fixtures (tests/golden/emd_small.npz, written by tests/golden/make_golden.py):
+the oracle must still reproduce them exactly (CPU), and the HIP path must match them to
+the parity bar (GPU) without running the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyemd_restated as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "emd_small.npz")
+
+
+def _load():
+    g = np.load(GOLD, allow_pickle=False)
+    off = np.zeros(len(g["lengths"]) + 1, np.int64)
+    off[1:] = np.cumsum(g["lengths"])
+    return g, off
+
+
+def _rows(g, k, off_rows):
+    n, L = int(g["n_imfs"][k]), int(g["lengths"][k])
+    return g["rows"][off_rows[k]:off_rows[k] + n * L].reshape(n, L)
+
+
+def _row_offsets(g):
+    ro = np.zeros(len(g["lengths"]) + 1, np.int64)
+    ro[1:] = np.cumsum(g["n_imfs"].astype(np.int64) * g["lengths"])
+    return ro
+
+
+def test_oracle_reproduces_golden():
+    g, off = _load()
+    ro = _row_offsets(g)
+    for k in range(len(g["lengths"])):
+        x = g["values"][off[k]:off[k + 1]]
+        r = O.emd(x, max_imf=15)
+        assert r.n_imfs == g["n_imfs"][k] and r.overflow == g["overflow"][k], g["names"][k]
+        assert np.array_equal(r.debug_n[:16], g["debug_n"][k])
+        assert np.array_equal(r.debug_extno[:16], g["debug_extno"][k])
+        assert np.array_equal(r.debug_nzm[:16], g["debug_nzm"][k])
+        assert np.array_equal(r.imfs, _rows(g, k, ro))
+
+
+@pytest.mark.gpu
+def test_gpu_matches_golden():
+    import torch
+    from cuda_emd import EMDKernelConfig
+    g, off = _load()
+    ro = _row_offsets(g)
+    launcher = EMDKernelConfig().build()
+    out = launcher.run_ragged(torch.from_numpy(g["values"]).cuda(), torch.from_numpy(off),
+                              threads=256, persistent=True)
+    imfs, oo = out[0].cpu().numpy(), out[1].cpu().numpy()
+    for k in range(len(g["lengths"])):
+        name = str(g["names"][k])
+        assert int(out[2][k]) == g["n_imfs"][k], name
+        assert int(out[3][k]) == g["overflow"][k], name
+        for j, key in ((4, "debug_n"), (5, "debug_extno"), (6, "debug_nzm")):
+            assert np.array_equal(out[j][k].cpu().numpy(), g[key][k]), (name, key)
+        n, L = int(g["n_imfs"][k]), int(g["lengths"][k])
+        got = imfs[oo[k]:oo[k] + n * L].reshape(n, L)
+        x = g["values"][off[k]:off[k + 1]]
+        scale = max(float(np.max(np.abs(x))), 1e-300) if L else 1.0
+        assert np.max(np.abs(got - _rows(g, k, ro)), initial=0.0) <= 1e-9 * scale, name
