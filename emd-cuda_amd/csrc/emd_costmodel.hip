@@ -55,20 +55,33 @@ __device__ __forceinline__ float gelu_grad(float z) {
 //   256-sample segments (one zero-padded segment when N < 256); bands over bins
 //   [1,2) [2,4) [4,8) [8,16) [16,32) [32,64) [64,96) [96,129).
 // ---------------------------------------------------------------------------------------
+__device__ __forceinline__ int bitrev8(int v) {
+  return (int)(__builtin_bitreverse32((unsigned)v) >> 24);
+}
+
+// Orders one wave's LDS accesses (its lanes' earlier writes are visible to later reads).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __global__ __launch_bounds__(256) void features_kernel(const double* __restrict__ values,
                                                         const int64_t* __restrict__ offsets,
                                                         float* __restrict__ feats) {
-  __shared__ double seg[kSeg];
-  __shared__ double cs[kSeg], sn[kSeg];
+  // each wave transforms its own segments: radix-2 FFT of 256 points in LDS, fp64
+  __shared__ double re[4][kSeg], im[4][kSeg];
+  __shared__ double cs[kSeg / 2], sn[kSeg / 2];
+  __shared__ double pw_w[4][kBins];
   __shared__ double pw[kBins];
   __shared__ double red[4][3];
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int64_t o = offsets[blockIdx.x];
   const int n = (int)(offsets[blockIdx.x + 1] - o);
   const double* x = values + o;
-  {
+  if (t < kSeg / 2) {
     double s, c;
-    sincospi((double)t / (kSeg / 2), &s, &c);
+    sincospi((double)t / (kSeg / 2), &s, &c);  // exp(-2 pi i t / 256) = c - i s
     cs[t] = c;
     sn[t] = s;
   }
@@ -87,35 +100,60 @@ __global__ __launch_bounds__(256) void features_kernel(const double* __restrict_
     zc += __shfl_down(zc, off);
     ss += __shfl_down(ss, off);
   }
-  if ((t & 63) == 0) {
-    red[t >> 6][0] = ext;
-    red[t >> 6][1] = zc;
-    red[t >> 6][2] = ss;
+  if (lane == 0) {
+    red[wv][0] = ext;
+    red[wv][1] = zc;
+    red[wv][2] = ss;
   }
+  __syncthreads();  // twiddles ready
   const int nseg = n >= kSeg ? n / kSeg : 1;
-  double acc = 0;
-  for (int sgi = 0; sgi < nseg; ++sgi) {
-    __syncthreads();
-    const int i = sgi * kSeg + t;
-    seg[t] = i < n ? x[i] : 0.0;
-    __syncthreads();
-    if (t < kBins) {
-      double re = 0, im = 0;
-      for (int j = 0; j < kSeg; ++j) {
-        const int q = (t * j) & (kSeg - 1);
-        re += seg[j] * cs[q];
-        im -= seg[j] * sn[q];
-      }
-      acc += re * re + im * im;
+  double p0 = 0.0, p1 = 0.0, p2 = 0.0;  // bins lane, lane + 64, 128
+  double* R = re[wv];
+  double* I = im[wv];
+  for (int sgi = wv; sgi < nseg; sgi += 4) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int j = lane + 64 * q;
+      const int i = sgi * kSeg + j;
+      R[bitrev8(j)] = i < n ? x[i] : 0.0;
+      I[bitrev8(j)] = 0.0;
     }
+    wave_sync();
+    for (int half = 1; half < kSeg; half <<= 1) {
+      const int step = kSeg / (2 * half);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int b = lane + 64 * q;  // butterfly 0..127
+        const int k = b & (half - 1);
+        const int i0 = (b - k) * 2 + k, i1 = i0 + half;
+        const double wr = cs[k * step], wi = -sn[k * step];
+        const double xr = R[i1], xi = I[i1];
+        const double tr = xr * wr - xi * wi, ti = xr * wi + xi * wr;
+        const double ur = R[i0], ui = I[i0];
+        R[i0] = ur + tr;
+        I[i0] = ui + ti;
+        R[i1] = ur - tr;
+        I[i1] = ui - ti;
+      }
+      wave_sync();
+    }
+    p0 += R[lane] * R[lane] + I[lane] * I[lane];
+    p1 += R[lane + 64] * R[lane + 64] + I[lane + 64] * I[lane + 64];
+    if (lane == 0) p2 += R[128] * R[128] + I[128] * I[128];
+    wave_sync();
   }
-  if (t < kBins) pw[t] = acc / nseg;
+  pw_w[wv][lane] = p0;
+  pw_w[wv][lane + 64] = p1;
+  if (lane == 0) pw_w[wv][128] = p2;
   __syncthreads();
+  if (t < kBins) pw[t] = (pw_w[0][t] + pw_w[1][t] + pw_w[2][t] + pw_w[3][t]) / nseg;
+  __syncthreads();
+  __shared__ double band[9];
   if (t < 9) {
     double e = 0;
     const int lo = t < 8 ? kBandEdge[t] : 0, hi = t < 8 ? kBandEdge[t + 1] : kBins;
     for (int k = lo; k < hi; ++k) e += pw[k];
-    seg[t] = e;  // seg[0..7] bands, seg[8] total
+    band[t] = e;  // band[0..7] bands, band[8] total
   }
   __syncthreads();
   if (t == 0) {
@@ -127,8 +165,8 @@ __global__ __launch_bounds__(256) void features_kernel(const double* __restrict_
     f[1] = (float)log1p(E);
     f[2] = (float)log1p(Z);
     f[3] = (float)log(sqrt(n > 0 ? S / n : 0.0) + 1e-12);
-    const double tot = seg[8];
-    for (int b = 0; b < 8; ++b) f[4 + b] = (float)log((tot > 0 ? seg[b] / tot : 0.0) + 1e-10);
+    const double tot = band[8];
+    for (int b = 0; b < 8; ++b) f[4 + b] = (float)log((tot > 0 ? band[b] / tot : 0.0) + 1e-10);
   }
 }
 
