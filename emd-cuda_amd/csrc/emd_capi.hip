@@ -115,7 +115,11 @@ int make_shape(const emd_handle* h, int wg, int tile_req, int max_n, Shape* s) {
   tile = std::max(0, std::min(tile, lds_fit_slots(h, wg, max_n)));
   s->wg = wg;
   s->tile = tile;
-  s->spill_slots = tile < cap ? cap : 0;
+  // A global region backs sifts whose knots outgrow the LDS tile, and holds the solver
+  // rows of systems longer than 64 * (kRegRows + 2) rows (a side has at most
+  // min(max_extrema, (max_n + 1) / 2) + 2 * nbsym knots).
+  const int max_rows = std::min(h->cfg.max_extrema, (max_n + 1) / 2) + 2 * h->cfg.nbsym;
+  s->spill_slots = (tile < cap || max_rows > 64 * (emdk::kRegRows + 2)) ? cap : 0;
   s->max_n = std::max(max_n, 1);
   return EMD_OK;
 }
@@ -248,7 +252,7 @@ struct Plan {
   emdk::SiftParams P;
   int persistent = 0;
   int grid = 0;
-  int64_t stride = 0;      // bytes per workgroup global tile (0: none needed)
+  int64_t stride = 0;      // bytes per workgroup global region (0: none needed)
   size_t spill_bytes = 0;  // stride x resident workgroups
 };
 
@@ -257,7 +261,7 @@ static int make_plan(const emd_handle* h, int B, int max_n, const emd_launch& L,
                       &pl->s);
   if (rc) return rc;
   pl->P = params(h, pl->s);
-  pl->stride = pl->s.spill_slots ? emdk::tile_bytes(pl->s.spill_slots) : 0;
+  pl->stride = pl->s.spill_slots ? emdk::global_region_bytes(pl->s.spill_slots) : 0;
   pl->persistent = L.persistent ? 1 : 0;
   if (pl->persistent) {
     const size_t lds = emdk::dyn_lds_bytes(pl->s.tile, pl->P.max_n);
@@ -400,7 +404,7 @@ int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double
   if (rc) return rc;
   const emdk::SiftParams P = params(h, s);
   const int mbuf = h->cfg.max_extrema + 2 * h->cfg.nbsym;
-  const int64_t stride = s.spill_slots ? emdk::tile_bytes(s.spill_slots) : 0;
+  const int64_t stride = s.spill_slots ? emdk::global_region_bytes(s.spill_slots) : 0;
   if (stride) {
     rc = ensure_spill(h, (size_t)stride * std::min(B, kMaxPlainBlocks));
     if (rc) return rc;

@@ -7,7 +7,9 @@ namespace emdk {
 
 constexpr int kMaxNbsym = 4;      // mirrored points per side supported by the kernels
 constexpr int kMaxN = 16384;      // longest trace (256 blocks of 64 samples)
-constexpr int kSlotBytes = 44;    // bytes per knot slot: 5 fp64 + 1 int32
+constexpr int kSlotBytes = 28;    // bytes per knot slot: position int32, value, M, 1/h fp64
+constexpr int kRegRows = 4;       // interior solver rows a lane keeps in registers
+constexpr int kRowBytesPerSlot = 64;  // global solver rows: 2 sides x 4 fp64 per knot slot
 constexpr int kWorkBlockBytes = 568;  // LDS bytes per 64-sample block of the work area
 
 struct SiftParams {
@@ -16,7 +18,9 @@ struct SiftParams {
   int max_iteration;  // sift cap per IMF
   int max_imfs;       // output rows per trace (IMF cap = max_imfs - 1, then the residue)
   int tile_slots;     // knot slots of the LDS tile (both envelopes)
-  int spill_slots;    // knot slots of each workgroup's global tile (holds any trace)
+  int spill_slots;    // knot slots of each workgroup's global region (0: none): a global
+                      // knot tile that holds any trace, plus solver rows for systems
+                      // longer than 64 * (kRegRows + 2) rows
   int max_n;          // longest trace of the launch (sizes the LDS work area)
   int zero_fill;      // write zeros into output rows at and beyond n_imfs
 };
@@ -64,6 +68,8 @@ struct EnvBatch {
 
 // Bytes of one workgroup's global knot tile of `slots` slots.
 int64_t tile_bytes(int slots);
+// bytes of one workgroup's global region for `slots` knot slots (knot tile + solver rows)
+int64_t global_region_bytes(int slots);
 // Static LDS of the sift kernels for a workgroup size.
 int static_lds_bytes(int wg);
 // Dynamic LDS of a launch: the per-trace work area (IMF + block bookkeeping) for traces of

@@ -40,27 +40,30 @@ constexpr double kTotalPowerThr = 0.005;
 constexpr double kRangeThr = 0.001;
 constexpr double kPowerFloor = 1e-10;
 constexpr double kResidueAtol = 1e-8;
+constexpr double kSixth = 1.0 / 6.0;
 constexpr int kWorkBytesPerBlock = 568;  // 64 fp64 + 4 fp64 + 2 u64 + 2 int per block
 
 enum SplineKind { kConst = 0, kLinear = 1, kThreePoint = 2, kNotAKnot = 3 };
 
 struct Tile {
   double* Y;   // knot values
-  double* A;   // sub-diagonal (rows); second derivatives M (by slot) after the solve
-  double* Bd;  // diagonal (rows)
-  double* Cc;  // super-diagonal (rows)
-  double* D;   // right-hand side -> solution (rows)
+  double* M;   // second derivatives by knot slot (after the solve)
+  double* H;   // 1/h of the interval that starts at each knot slot (after the solve)
   int* X;      // knot positions (sample index; mirrored knots may be <0 or >N-1)
+  double* R;   // solver rows of systems longer than 64 * (kRegRows + 2) rows, in the
+               // workgroup's global region: side s, array a (g, u, v, w) at
+               // R + (4 * s + a) * rows; null when the launch never needs them
+  int rows;
 };
 
 __device__ __forceinline__ Tile make_tile(double* base, int slots) {
   Tile t;
   t.Y = base;
-  t.A = base + slots;
-  t.Bd = base + 2 * slots;
-  t.Cc = base + 3 * slots;
-  t.D = base + 4 * slots;
-  t.X = reinterpret_cast<int*>(base + 5 * slots);
+  t.M = base + slots;
+  t.H = base + 2 * slots;
+  t.X = reinterpret_cast<int*>(base + 3 * slots);
+  t.R = nullptr;
+  t.rows = 0;
   return t;
 }
 
@@ -74,7 +77,7 @@ struct SiftShared {
   int app_w[NW];
   int tot_max, tot_min;
   // mirror results
-  int k0[2], nk[2], kind[2], row0[2], mtot;
+  int k0[2], nk[2], kind[2], mtot;
   int bad;
   double p3[2][4];
   int queue_idx;
@@ -663,9 +666,10 @@ __device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int
 // every interior unknown as x = u - v*x_s - w*x_e; the chunk end rows then form a
 // tridiagonal system in (x_s, x_e) over all chunks, which an odd-even step reduces to
 // one unknown per lane and parallel cyclic reduction across lanes solves with shuffles.
-// The systems are diagonally dominant, so no pivoting is needed.  The solution is left
-// in t.D by row; the end second derivatives are then recovered and M is stored in t.A
-// by knot slot.
+// The systems are diagonally dominant, so no pivoting is needed.  A lane keeps its chunk
+// rows in registers (up to kRegRows rows; longer systems keep them in the workgroup's
+// global region).  The solution goes to M by knot slot; the end second derivatives are
+// then recovered and 1/h is stored per interval.
 // =======================================================================================
 // Row j of the system of a side whose knots start at slot k0: a x_{j-1} + b x_j + c x_{j+1} = d.
 __device__ __forceinline__ void nak_row(const Tile& t, int k0, int m, int j, double& a, double& b,
@@ -693,59 +697,100 @@ __device__ __forceinline__ void nak_row(const Tile& t, int k0, int m, int j, dou
   }
 }
 
-// Solve one side's system (rows r0 .. r0+m-1 of the tile's row arrays) with the calling
-// wave; every lane of the wave must call it.
-__device__ __forceinline__ void spline_system_wave(const Tile& t, int k0, int m, int r0,
+// One interior row of the chunk sweep: updates the carried (g, u, v, w) of row j from
+// those of row j-1.
+__device__ __forceinline__ void sweep_row(const Tile& t, int k0, int m, int j, bool first,
+                                          bool last, double& g, double& u, double& v,
+                                          double& w) {
+  double a, b, c, d;
+  nak_row(t, k0, m, j, a, b, c, d);
+  const double rb = 1.0 / (first ? b : b - a * g);
+  const double gn = last ? 0.0 : c * rb;
+  const double un = (first ? d : d - a * u) * rb;
+  const double vn = (first ? a : -(a * v)) * rb;
+  const double wn = ((last ? c : 0.0) - (first ? 0.0 : a * w)) * rb;
+  g = gn;
+  u = un;
+  v = vn;
+  w = wn;
+}
+
+// Solve one side's system (knots from slot k0, m rows) with the calling wave; every lane
+// of the wave must call it.  Writes M[k0+1 .. k0+m].
+__device__ __forceinline__ void spline_system_wave(const Tile& t, int side, int k0, int m,
                                                    int lane) {
   const int K = min(64, m / 2);  // chunks of >= 2 rows
   const int base = m / K, rem = m % K;
+  const bool in_regs = base + (rem > 0 ? 1 : 0) - 2 <= kRegRows;  // interior rows per lane
   const bool act = lane < K;
   const int s = act ? lane * base + min(lane, rem) : 0;
   const int e = act ? s + base + (lane < rem ? 1 : 0) - 1 : 0;
+  const int len = act ? e - s - 1 : 0;  // interior rows of this chunk
+  double rg[kRegRows], ru[kRegRows], rv[kRegRows], rw[kRegRows];
+  double* Rg = t.R ? t.R + (4 * side + 0) * t.rows : nullptr;
+  double* Ru = t.R ? t.R + (4 * side + 1) * t.rows : nullptr;
+  double* Rv = t.R ? t.R + (4 * side + 2) * t.rows : nullptr;
+  double* Rw = t.R ? t.R + (4 * side + 3) * t.rows : nullptr;
   // chunk end rows after eliminating the interior (identity rows on idle lanes)
   double rsa = 0.0, rsb = 1.0, rsc = 0.0, rsd = 0.0;
   double rea = 0.0, reb = 1.0, rec = 0.0, red = 0.0;
   if (act) {
     double a, b, c, d;
     double us = 0.0, vs = 0.0, ws = 0.0, ue = 0.0, ve = 0.0, we = 0.0;
-    const bool interior = e - s >= 2;
-    if (interior) {
-      // forward sweep: Bd = g (modified super-diagonal), D = u, A = v, Cc = w
-      double gp = 0.0, up = 0.0, vp = 0.0, wp = 0.0;
-      for (int j = s + 1; j < e; ++j) {
-        nak_row(t, k0, m, j, a, b, c, d);
-        const bool first = j == s + 1, last = j == e - 1;
-        const double rb = 1.0 / (first ? b : b - a * gp);
-        const double g = last ? 0.0 : c * rb;
-        const double u = (first ? d : d - a * up) * rb;
-        const double v = (first ? a : -(a * vp)) * rb;
-        const double w = ((last ? c : 0.0) - (first ? 0.0 : a * wp)) * rb;
-        t.Bd[r0 + j] = g;
-        t.D[r0 + j] = u;
-        t.A[r0 + j] = v;
-        t.Cc[r0 + j] = w;
-        gp = g;
-        up = u;
-        vp = v;
-        wp = w;
+    if (len > 0) {
+      double g = 0.0, u = 0.0, v = 0.0, w = 0.0;
+      if (in_regs) {
+#pragma unroll
+        for (int q = 0; q < kRegRows; ++q) {
+          if (q < len) {
+            sweep_row(t, k0, m, s + 1 + q, q == 0, q == len - 1, g, u, v, w);
+            rg[q] = g;
+            ru[q] = u;
+            rv[q] = v;
+            rw[q] = w;
+          }
+        }
+        ue = u;
+        ve = v;
+        we = w;
+#pragma unroll
+        for (int q = kRegRows - 2; q >= 0; --q) {
+          if (q < len - 1) {
+            ru[q] = ru[q] - rg[q] * ru[q + 1];
+            rv[q] = rv[q] - rg[q] * rv[q + 1];
+            rw[q] = rw[q] - rg[q] * rw[q + 1];
+          }
+        }
+        us = ru[0];
+        vs = rv[0];
+        ws = rw[0];
+      } else {
+        for (int q = 0; q < len; ++q) {
+          sweep_row(t, k0, m, s + 1 + q, q == 0, q == len - 1, g, u, v, w);
+          Rg[s + 1 + q] = g;
+          Ru[s + 1 + q] = u;
+          Rv[s + 1 + q] = v;
+          Rw[s + 1 + q] = w;
+        }
+        ue = u;
+        ve = v;
+        we = w;
+        for (int q = len - 2; q >= 0; --q) {
+          const int j = s + 1 + q;
+          const double gq = Rg[j];
+          u = Ru[j] - gq * u;
+          v = Rv[j] - gq * v;
+          w = Rw[j] - gq * w;
+          Ru[j] = u;
+          Rv[j] = v;
+          Rw[j] = w;
+        }
+        us = u;
+        vs = v;
+        ws = w;
       }
-      ue = up;
-      ve = vp;
-      we = wp;
-      // backward sweep
-      for (int j = e - 2; j > s; --j) {
-        const double g = t.Bd[r0 + j];
-        up = t.D[r0 + j] - g * up;
-        vp = t.A[r0 + j] - g * vp;
-        wp = t.Cc[r0 + j] - g * wp;
-        t.D[r0 + j] = up;
-        t.A[r0 + j] = vp;
-        t.Cc[r0 + j] = wp;
-      }
-      us = up;
-      vs = vp;
-      ws = wp;
     }
+    const bool interior = len > 0;
     nak_row(t, k0, m, s, a, b, c, d);
     rsa = a;
     rsb = interior ? b - c * vs : b;
@@ -789,10 +834,18 @@ __device__ __forceinline__ void spline_system_wave(const Tile& t, int k0, int m,
   const double xprev = __shfl_up(xe, 1);
   if (act) {
     const double xs = (rsd - (lane > 0 ? rsa * xprev : 0.0) - rsc * xe) / rsb;
-    t.D[r0 + s] = xs;
-    t.D[r0 + e] = xe;
-    for (int j = s + 1; j < e; ++j)
-      t.D[r0 + j] = t.D[r0 + j] - t.A[r0 + j] * xs - t.Cc[r0 + j] * xe;
+    t.M[k0 + 1 + s] = xs;
+    t.M[k0 + 1 + e] = xe;
+    if (in_regs) {
+#pragma unroll
+      for (int q = 0; q < kRegRows; ++q)
+        if (q < len) t.M[k0 + 2 + s + q] = ru[q] - rv[q] * xs - rw[q] * xe;
+    } else {
+      for (int q = 0; q < len; ++q) {
+        const int j = s + 1 + q;
+        t.M[k0 + 1 + j] = Ru[j] - Rv[j] * xs - Rw[j] * xe;
+      }
+    }
   }
 }
 
@@ -800,45 +853,34 @@ template <int WG>
 __device__ __forceinline__ void spline_solve(const Tile& t, const SiftShared<WG>& sh) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int mtot = uni(sh.mtot);
-  const int row1 = uni(sh.row0[1]);
   if (mtot > 0) {
     // wave 0 solves the maxima envelope and wave 1 the minima (wave 0 both when WG = 64);
-    // their row ranges are disjoint
+    // their knot slots are disjoint
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
       const int owner = WG >= 128 ? side : 0;
       if (wv == owner && uni(sh.kind[side]) == kNotAKnot)
-        spline_system_wave(t, uni(sh.k0[side]), uni(sh.nk[side]) - 2, uni(sh.row0[side]), lane);
+        spline_system_wave(t, side, uni(sh.k0[side]), uni(sh.nk[side]) - 2, lane);
     }
     __syncthreads();
-    // scatter M by knot slot into A (A's row contents are dead now)
-    for (int r = tid; r < mtot; r += WG) {
-      const int side = (r < row1) ? 0 : 1;
-      const int k = r - sh.row0[side];
-      t.A[sh.k0[side] + k + 1] = t.D[r];
-    }
     const int side = (tid == 0) ? 0 : (tid == 1 ? 1 : -1);
     if (side >= 0 && sh.kind[side] == kNotAKnot) {
-      const int n = sh.nk[side], k0 = sh.k0[side], r0 = sh.row0[side], m = n - 2;
+      const int n = sh.nk[side], k0 = sh.k0[side];
       // all loads first, so they are in flight together
       const double x0 = t.X[k0], x1 = t.X[k0 + 1], x2 = t.X[k0 + 2];
-      const double M1 = t.D[r0], M2 = t.D[r0 + 1];
+      const double M1 = t.M[k0 + 1], M2 = t.M[k0 + 2];
       const double xa = t.X[k0 + n - 3], xb = t.X[k0 + n - 2], xc = t.X[k0 + n - 1];
-      const double Mb = t.D[r0 + m - 1], Ma = t.D[r0 + m - 2];
+      const double Mb = t.M[k0 + n - 2], Ma = t.M[k0 + n - 3];
       const double h0 = x1 - x0, h1 = x2 - x1;
       const double hnm3 = xb - xa, hnm2 = xc - xb;
-      t.A[k0] = ((h0 + h1) * M1 - h0 * M2) / h1;
-      t.A[k0 + n - 1] = ((hnm3 + hnm2) * Mb - hnm2 * Ma) / hnm3;
+      t.M[k0] = ((h0 + h1) * M1 - h0 * M2) / h1;
+      t.M[k0 + n - 1] = ((hnm3 + hnm2) * Mb - hnm2 * Ma) / hnm3;
     }
-    // per-interval constants by knot slot: Bd = 1/h, Cc = h*h/6 (Bd/Cc rows are dead)
-    for (int side = 0; side < 2; ++side) {
-      if (sh.kind[side] != kNotAKnot) continue;
-      const int k0 = sh.k0[side], n = sh.nk[side];
-      for (int k = tid; k < n - 1; k += WG) {
-        const double h = double(t.X[k0 + k + 1] - t.X[k0 + k]);
-        t.Bd[k0 + k] = 1.0 / h;
-        t.Cc[k0 + k] = h * h / 6.0;
-      }
+    // 1/h per interval by knot slot
+    for (int sd = 0; sd < 2; ++sd) {
+      if (sh.kind[sd] != kNotAKnot) continue;
+      const int k0 = sh.k0[sd], n = sh.nk[sd];
+      for (int k = tid; k < n - 1; k += WG) t.H[k0 + k] = 1.0 / double(t.X[k0 + k + 1] - t.X[k0 + k]);
     }
   }
   __syncthreads();
@@ -888,10 +930,12 @@ __device__ __forceinline__ double envelope_at(const Tile& t, const SiftShared<WG
   const int k0 = sh.k0[side];
   if (kind != kNotAKnot) return eval_small(t, k0, kind, sh.p3[side], i);
   const int j = interval_of(t, sh, side, wk, blk, lane, i, tb) + k0;
-  const double A = double(t.X[j + 1] - i) * t.Bd[j];
-  const double B = double(i - t.X[j]) * t.Bd[j];
+  const int x0 = t.X[j], x1 = t.X[j + 1];
+  const double h = double(x1 - x0);
+  const double A = double(x1 - i) * t.H[j];
+  const double B = double(i - x0) * t.H[j];
   return A * t.Y[j] + B * t.Y[j + 1] +
-         ((A * A * A - A) * t.A[j] + (B * B * B - B) * t.A[j + 1]) * t.Cc[j];
+         ((A * A * A - A) * t.M[j] + (B * B * B - B) * t.M[j + 1]) * (h * h * kSixth);
 }
 
 // Perfect binary tree over nblk block partials (zero-padded to a power of two), computed
@@ -948,11 +992,13 @@ __device__ __forceinline__ void envelope_pair(const Tile& t, const SiftShared<WG
       xb = t.X[k0 + j + 1];
     }
     const int jj = k0 + j;
-    const double h1 = t.Bd[jj];
-    const double A = double(xb - ii) * h1;
-    const double B = double(ii - xa) * h1;
+    const double ih = t.H[jj];
+    const double hh = double(xb - xa);
+    const double A = double(xb - ii) * ih;
+    const double B = double(ii - xa) * ih;
     const double e = A * t.Y[jj] + B * t.Y[jj + 1] +
-                     ((A * A * A - A) * t.A[jj] + (B * B * B - B) * t.A[jj + 1]) * t.Cc[jj];
+                     ((A * A * A - A) * t.M[jj] + (B * B * B - B) * t.M[jj + 1]) *
+                         (hh * hh * kSixth);
     if (q == 0) e0 = e; else e1 = e;
   }
 }
@@ -1081,8 +1127,6 @@ __device__ __forceinline__ void build_envelopes(const Tile& t, SiftShared<WG>& s
       }
       const int m0 = kind[0] == kNotAKnot ? nk[0] - 2 : 0;
       const int m1 = kind[1] == kNotAKnot ? nk[1] - 2 : 0;
-      sh.row0[0] = 0;
-      sh.row0[1] = m0;
       sh.mtot = m0 + m1;
     }
   }
@@ -1324,14 +1368,22 @@ __device__ __forceinline__ WgMem workgroup_mem(double* lds, char* spill, int64_t
   const int nbcap = (P.max_n + 63) / 64;
   m.wk = make_work(lds, nbcap);
   m.tl = make_tile(lds + (int64_t)nbcap * (kWorkBytesPerBlock / 8), P.tile_slots);
-  m.tg = make_tile(reinterpret_cast<double*>(spill + (int64_t)blockIdx.x * stride),
-                   P.spill_slots);
+  char* region = spill + (int64_t)blockIdx.x * stride;
+  m.tg = make_tile(reinterpret_cast<double*>(region), P.spill_slots);
+  if (P.spill_slots > 0) {
+    // solver rows after the knot tile (tile_bytes rounding: a multiple of 256 bytes)
+    const int64_t tb = ((int64_t)P.spill_slots * kSlotBytes + 255) / 256 * 256;
+    m.tg.R = reinterpret_cast<double*>(region + tb);
+    m.tg.rows = P.spill_slots;
+  }
+  m.tl.R = m.tg.R;
+  m.tl.rows = m.tg.rows;
   return m;
 }
 
 // One workgroup per trace (grid = traces of this launch).  The reference launch model.
 template <int WG>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void emd_plain_kernel(Batch b, SiftParams P) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void emd_plain_kernel(Batch b, SiftParams P) {
   extern __shared__ double dyn_lds[];
   __shared__ SiftShared<WG> sh;
   const WgMem m = workgroup_mem(dyn_lds, b.spill, b.spill_stride, P);
@@ -1343,7 +1395,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void em
 // Persistent workgroups pull queue positions from an atomic counter until the b.B queue
 // entries are drained.
 template <int WG>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void emd_persistent_kernel(Batch b, SiftParams P) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void emd_persistent_kernel(Batch b, SiftParams P) {
   extern __shared__ double dyn_lds[];
   __shared__ SiftShared<WG> sh;
   const WgMem m = workgroup_mem(dyn_lds, b.spill, b.spill_stride, P);
@@ -1361,7 +1413,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void em
 
 // Envelope pass without sifting (diagnostic; reference envelope_batch_kernel).
 template <int WG>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void emd_envelope_kernel(EnvBatch e, SiftParams P) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void emd_envelope_kernel(EnvBatch e, SiftParams P) {
   extern __shared__ double dyn_lds[];
   __shared__ SiftShared<WG> sh;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;

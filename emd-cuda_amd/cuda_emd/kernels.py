@@ -61,7 +61,8 @@ class EMDKernelConfig:
 
     def shared_mem_bytes(self) -> int:
         """Dynamic LDS of the knot tile: slots for both envelopes x bytes per slot
-        (5 fp64 + 1 int32).  Reference counterpart: kernels.py:1273-1275."""
+        (position int32; value, second derivative, 1/h fp64).  Reference counterpart:
+        kernels.py:1273-1275."""
         slots = self.tile_slots or (2 * self.max_extrema + 4 * self.nbsym)
         return slots * int(_lib.load().emd_lds_slot_bytes())
 

@@ -29,6 +29,6 @@ def test_prototypes_match_header():
 def test_host_only_queries():
     lib = _lib.load()
     assert lib.emd_version() >= 1
-    assert lib.emd_lds_slot_bytes() == 44
+    assert lib.emd_lds_slot_bytes() == 28
     for wg in (64, 128, 256):
         assert 0 < lib.emd_static_lds_bytes(wg) < 4096
