@@ -420,14 +420,16 @@ def main():
         doff = offsets.cuda()
         plan_tile = []
 
+        dlen = torch.from_numpy(lengths.astype(np.float64)).cuda()
+
         def packed():
             # the whole learned path: features -> MLP -> queue order + tile -> sift launch
             f = costmodel.features(dx.view(-1), doff)
-            plan = cm.plan(f, lengths, nbsym=cfg.nbsym)
+            plan = cm.plan(f, dlen, nbsym=cfg.nbsym)
             tile = args.tile or plan.tile_slots
             plan_tile.append(tile)
-            return launcher.run_ragged(dx.view(-1), doff, order=plan.order, threads=args.threads,
-                                       tile_slots=tile, persistent=True)
+            return launcher.run_ragged(dx.view(-1), offsets, order=plan.order,
+                                       threads=args.threads, tile_slots=tile, persistent=True)
         out, ms_wall, ms_ev = timed(packed)
         results["packed"] = dict(ms=ms_wall, kernel_ms=ms_ev, n_imfs=out[2].cpu().numpy(),
                                  stats=out[7].cpu().numpy(), out=out)

@@ -17,7 +17,7 @@ in batch order (kernels.py:1214-1220) and sizes LDS from max_extrema (kernels.py
 from __future__ import annotations
 
 import ctypes
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -82,7 +82,6 @@ def mape(pred, true) -> dict:
 class Plan:
     order: object            # int32 device tensor, queue order (LPT)
     tile_slots: int          # LDS knot slots
-    predicted: np.ndarray = field(repr=False, default=None)  # [B,3] raw predictions
 
 
 class CostModel:
@@ -171,18 +170,27 @@ class CostModel:
     def plan(self, feats, lengths, nbsym: int = 2, quantile: float = 0.995,
              margin: float = 1.1, cap: int | None = None) -> Plan:
         """Queue order (longest predicted sifts x N first) and LDS knot tile (both
-        envelopes of a sift at the ``quantile`` of predicted peak extrema, x margin)."""
+        envelopes of a sift at the ``quantile`` of predicted peak extrema, x margin).
+        Everything stays on the device except the one tile size the launch needs."""
         torch = _torch()
-        pred = self.predict(feats)
-        L = np.asarray(lengths, dtype=np.float64)
-        cost = np.maximum(pred[:, 0], 1.0) * L
-        order = torch.from_numpy(np.argsort(-cost, kind="stable").astype(np.int32)).to(self.device)
-        peak = np.quantile(np.maximum(pred[:, 2], 0.0), quantile) if len(pred) else 0.0
+        y = self.forward_std(feats).double()
+        tsd = torch.tensor(self.tsd, dtype=torch.float64, device=self.device)
+        tmu = torch.tensor(self.tmu, dtype=torch.float64, device=self.device)
+        y = y * tsd + tmu
+        L = lengths if isinstance(lengths, torch.Tensor) else torch.from_numpy(
+            np.asarray(lengths, dtype=np.float64))
+        L = L.to(self.device, torch.float64)
+        cost = torch.expm1(y[:, 0]).clamp_min(1.0) * L
+        order = torch.argsort(cost, descending=True, stable=True).to(torch.int32)
+        if y.shape[0]:
+            peak = float(torch.quantile(torch.expm1(y[:, 2]).clamp_min(0.0), quantile))
+        else:
+            peak = 0.0
         tile = int(2 * np.ceil(peak * margin) + 4 * nbsym)
         tile = max(tile, 16 + 4 * nbsym)
         if cap is not None:
             tile = min(tile, cap)
-        return Plan(order, tile, pred)
+        return Plan(order, tile)
 
     # -- persistence -------------------------------------------------------------------------
     def state(self) -> dict:
