@@ -7,9 +7,10 @@ stream, so the tail of one bucket overlaps the next.  Per bucket:
 
 * the cost model (when given) orders the queue longest-predicted-first (sifts x N) and
   proposes the LDS knot tile (a high quantile of predicted peak extrema);
-* the autotuner picks the workgroup size (64/128/256), the LDS tile (the model's proposal
-  or the capacity rule) and the resident workgroups per CU by timing the candidates on a
-  sample of the bucket's traces; results are cached in a JSON file.
+* the autotuner picks the workgroup size (64/128/256), the LDS tile (the capacity rule, the
+  model's proposal, or the largest tile that lets 2, 3 or 4 workgroups share a CU) and the
+  resident workgroups per CU by timing the candidates on a sample of the bucket's traces;
+  results are cached in a JSON file.
 
 None of these choices changes a decomposition (canonical arithmetic), which the tests
 check.  No reference counterpart: the reference needs one shared N per launch and launches
@@ -77,6 +78,20 @@ class Autotuner:
         return (f"{_device_key()}|me{c.max_extrema}|nb{c.nbsym}|it{c.max_iteration}|"
                 f"imfs{c.max_imfs}|b{BUCKET_BOUNDS[bucket]}|tile{tile_hint}")
 
+    @staticmethod
+    def _largest_tile_for(launcher, threads: int, max_n: int, k: int, lo: int = 32,
+                          hi: int = 8192) -> int:
+        """Largest LDS tile (slots) with at least k resident workgroups per CU, or 0."""
+        if launcher.max_resident(threads, lo, max_n) < k:
+            return 0
+        while hi - lo > 8:
+            mid = (lo + hi) // 2
+            if launcher.max_resident(threads, mid, max_n) >= k:
+                lo = mid
+            else:
+                hi = mid
+        return lo
+
     def choose(self, launcher: EMDBatchLauncher, values, offsets_host: np.ndarray,
                idx: np.ndarray, bucket: int, tile_hint: int = 0) -> LaunchChoice:
         """Best launch for the traces ``idx`` of bucket ``bucket`` (tile_hint: the cost
@@ -94,12 +109,20 @@ class Autotuner:
         gather = np.concatenate([np.arange(offsets_host[s], offsets_host[s + 1]) for s in pick])
         sub_vals = values[torch.from_numpy(gather).to(values.device)].contiguous()
         sub_off_t = torch.from_numpy(sub_off).cuda()
-        tiles = [0] + ([tile_hint] if tile_hint else [])
+        max_n = int(lens.max())
         launcher.set_timing(True)
         best = None
         for th in self.threads:
+            # LDS budgets: the capacity rule, the cost model's proposal, and the largest tile
+            # that still lets k workgroups share a CU (sifts that outgrow it use the
+            # workgroup's global tile)
+            tiles = [0] + ([tile_hint] if tile_hint else [])
+            for k in (2, 3, 4):
+                t = self._largest_tile_for(launcher, th, max_n, k)
+                if t and t not in tiles:
+                    tiles.append(t)
             for tile in tiles:
-                fit = launcher.max_resident(th, tile, int(lens.max()), persistent=True)
+                fit = launcher.max_resident(th, tile, max_n, persistent=True)
                 for wpc in sorted({fit, max(1, fit - 1)}, reverse=True):
                     ms = []
                     for _ in range(2):
