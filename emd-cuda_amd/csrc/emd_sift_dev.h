@@ -75,6 +75,7 @@ struct SiftShared {
   int plat_w[NW];
   int bad_w[NW];
   int app_w[NW];
+  int tot_max, tot_min;
   // mirror results
   int k0[2], nk[2], kind[2], mtot;
   int bad;
@@ -260,40 +261,18 @@ __device__ __forceinline__ void serial_extrema(const double* S, int N, bool want
   // a run still open here reaches the end of the signal and is dropped
 }
 
-// Exclusive prefix (maxima | minima << 16) of the extrema counts of blocks lane*per + k,
-// held by every lane of every wave after detect's scan.
-struct BlockPrefix {
-  unsigned pre[4];
-  int per;
-};
-
-__device__ __forceinline__ unsigned block_prefix(const BlockPrefix& bp, int blk) {
-  const int k = blk % bp.per;  // wave-uniform
-  const unsigned v = k == 0 ? bp.pre[0] : k == 1 ? bp.pre[1] : k == 2 ? bp.pre[2] : bp.pre[3];
-  return (unsigned)__builtin_amdgcn_readlane((int)v, blk / bp.per);
-}
-
-// Pass B: scatter each block's extrema into the knot tile (the block prefix also goes to
-// LDS for the envelope update's interval lookup).
 template <int WG>
 __device__ __forceinline__ void compact_knots(const Work& wk, int N, const Tile& t, int nm,
-                                              int nn, int pad, const BlockPrefix& bp,
-                                              int& bad) {
+                                              int nn, int pad, int& bad) {
   constexpr int NW = WG / 64;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nblk = ceil_div(N, 64);
   const int tb0 = pad, tb1 = 3 * pad + nm;
   for (int blk = w; blk < nblk; blk += NW) {
     const int i = blk * 64 + lane;
-    const unsigned pb = block_prefix(bp, blk);
-    const int pmx = int(pb & 0xffffu), pmn = int(pb >> 16);
-    if (lane == 0) {
-      wk.pmax[blk] = pmx;
-      wk.pmin[blk] = pmn;
-    }
     const uint64_t bm = wk.mmax[blk], bn = wk.mmin[blk];
     if ((bm >> lane) & 1ull) {
-      const int k = pmx + __popcll(bm & lanemask_lt(lane));
+      const int k = wk.pmax[blk] + __popcll(bm & lanemask_lt(lane));
       if (k < nm) {
         const double y = wk.imf[i];
         t.X[tb0 + k] = i;
@@ -302,7 +281,7 @@ __device__ __forceinline__ void compact_knots(const Work& wk, int N, const Tile&
       }
     }
     if ((bn >> lane) & 1ull) {
-      const int k = pmn + __popcll(bn & lanemask_lt(lane));
+      const int k = wk.pmin[blk] + __popcll(bn & lanemask_lt(lane));
       if (k < nn) {
         const double y = wk.imf[i];
         t.X[tb1 + k] = i;
@@ -376,33 +355,34 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
     }
     __syncthreads();
   }
-  // ---- exclusive prefix of per-block counts: every wave scans all blocks itself (up to
-  //      4 blocks per lane), so pass B needs no further barrier --------------------------
-  const int per = ceil_div(nblk, 64);
-  unsigned cnt[4] = {0u, 0u, 0u, 0u};
-  unsigned loc = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int b = lane * per + k;
-    if (k < per && b < nblk) {
-      cnt[k] = __popcll(wk.mmax[b]) | (unsigned(__popcll(wk.mmin[b])) << 16);
-      loc += cnt[k];
+  // ---- exclusive prefix of per-block counts (wave 0; up to 4 blocks per lane) ---------
+  if (w == 0) {
+    const int per = ceil_div(nblk, 64);
+    unsigned loc = 0;
+    for (int k = 0; k < per; ++k) {
+      const int b = lane * per + k;
+      if (b < nblk) loc += __popcll(wk.mmax[b]) | (unsigned(__popcll(wk.mmin[b])) << 16);
+    }
+    const unsigned incl = wave_incl_scan(loc, lane);
+    unsigned run = incl - loc;
+    for (int k = 0; k < per; ++k) {
+      const int b = lane * per + k;
+      if (b < nblk) {
+        wk.pmax[b] = int(run & 0xffffu);
+        wk.pmin[b] = int(run >> 16);
+        run += __popcll(wk.mmax[b]) | (unsigned(__popcll(wk.mmin[b])) << 16);
+      }
+    }
+    if (lane == 63) {
+      sh.tot_max = int(incl & 0xffffu);
+      sh.tot_min = int(incl >> 16);
     }
   }
-  const unsigned incl = wave_incl_scan(loc, lane);
-  BlockPrefix bp;
-  bp.per = per;
-  unsigned run = incl - loc;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    bp.pre[k] = run;
-    run += cnt[k];
-  }
-  const unsigned tot = (unsigned)__builtin_amdgcn_readlane((int)incl, 63);
+  __syncthreads();
   DetectResult R;
   R.nzer = uni(nzt);
-  R.tm = int(tot & 0xffffu);
-  R.tn = int(tot >> 16);
+  R.tm = uni(sh.tot_max);
+  R.tn = uni(sh.tot_min);
   R.nm = min(R.tm, P.max_extrema);
   R.nn = min(R.tn, P.max_extrema);
   R.ovf = (R.tm >= P.max_extrema || R.tn >= P.max_extrema);
@@ -410,8 +390,8 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
 
   // ---- pass B: compact the extrema into the knot tile ---------------------------------
   int bad = 0;
-  if (R.in_lds) compact_knots<WG>(wk, N, tl, R.nm, R.nn, pad, bp, bad);
-  else compact_knots<WG>(wk, N, tg, R.nm, R.nn, pad, bp, bad);
+  if (R.in_lds) compact_knots<WG>(wk, N, tl, R.nm, R.nn, pad, bad);
+  else compact_knots<WG>(wk, N, tg, R.nm, R.nn, pad, bad);
   const int bw = (__ballot(bad) != 0ull) ? 1 : 0;
   if (lane == 0) sh.bad_w[w] = bw;
   __syncthreads();
