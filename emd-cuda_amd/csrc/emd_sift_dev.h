@@ -80,6 +80,7 @@ struct SiftShared {
   int k0[2], nk[2], kind[2], mtot;
   int bad;
   double p3[2][4];
+  double carry[2][3];  // sum of squares, max, min of the IMF left by sift n (slot n & 1)
   int queue_idx;
 };
 
@@ -146,12 +147,15 @@ constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
 constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
 constexpr int kDppHalfMirror = 0x141; // row_half_mirror
 constexpr int kDppMirror = 0x140;     // row_mirror
+constexpr int kDppBcast15 = 0x142;    // row_bcast:15 (lane 15 of each row -> the next row)
+constexpr int kDppBcast31 = 0x143;    // row_bcast:31 (lane 31 -> rows 2 and 3)
 
-template <int CTRL>
+// Lanes of rows outside ROWS keep 0 (the `old` operand).
+template <int CTRL, int ROWS = 0xF>
 __device__ __forceinline__ double dpp_d(double v) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWS, 0xF, false);
   return __longlong_as_double(
       (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo));
 }
@@ -179,6 +183,17 @@ __device__ __forceinline__ double wave_tree_sum(double v) {
   v += dpp_d<kDppHalfMirror>(v);
   v += dpp_d<kDppMirror>(v);
   return (lane_d(v, 0) + lane_d(v, 16)) + (lane_d(v, 32) + lane_d(v, 48));
+}
+// The same tree with the last two levels as row broadcasts; only lane 63 holds the sum
+// (row 3 adds row 2's value, then rows 0+1 from lane 31: ((r0+r1)+(r2+r3)) bit for bit).
+__device__ __forceinline__ double wave_tree_sum_last(double v) {
+  v += dpp_d<kDppXor1>(v);
+  v += dpp_d<kDppXor2>(v);
+  v += dpp_d<kDppHalfMirror>(v);
+  v += dpp_d<kDppMirror>(v);
+  v += dpp_d<kDppBcast15, 0xA>(v);
+  v += dpp_d<kDppBcast31, 0x8>(v);
+  return v;
 }
 __device__ __forceinline__ double wave_max(double v) {
   v = fmax(v, dpp_d<kDppXor1>(v));
@@ -1008,19 +1023,23 @@ __device__ __forceinline__ void envelope_pair(const Tile& t, const SiftShared<WG
 // adjacent samples of a 128-sample block; the pair is added first, then the 64-lane
 // tree, then the tree over blocks -- the same perfect binary tree over next_pow2(N)
 // samples (zero-padded) as the 64-sample blocks of the other reductions.
+// Sift n's imf_old is the IMF that sift n-1 left, bit for bit (the detection in between
+// does not touch it), so its sum of squares (the same tree as sq_new) and its range come
+// from sh.carry[(n-1)&1] instead of being recomputed; the IMF's start pass fills carry[0]
+// for n == 1.
 template <int WG>
 __device__ __forceinline__ SiftStats envelope_update(const Work& wk, const Tile& t,
                                                      SiftShared<WG>& sh, int N,
-                                                     const DetectResult& D, int nb) {
+                                                     const DetectResult& D, int nb, int n) {
   constexpr int NW = WG / 64;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int nblk2 = ceil_div(N, 128);
   const int tb0 = nb, tb1 = 3 * nb + D.nm;
   const int kind0 = uni(sh.kind[0]), kind1 = uni(sh.kind[1]);
-  double omax = -INFINITY, omin = INFINITY;
+  double nmax = -INFINITY, nmin = INFINITY;
   for (int c = w; c < nblk2; c += NW) {
     const int i = c * 128 + 2 * lane;
-    double ln = 0.0, ld = 0.0, lo = 0.0, ls = 0.0;
+    double ln = 0.0, ld = 0.0, ls = 0.0;
     if (i < N) {
       const int b = 2 * c + (lane >> 5), p = (2 * lane) & 63;
       double emax0, emax1, emin0, emin1;
@@ -1043,10 +1062,9 @@ __device__ __forceinline__ SiftStats envelope_update(const Work& wk, const Tile&
       const double r0 = d0 / nw0;
       ln = nw0 * nw0;
       ld = d0 * d0;
-      lo = old0 * old0;
       ls = r0 * r0;
-      omax = fmax(omax, old0);
-      omin = fmin(omin, old0);
+      nmax = fmax(nmax, nw0);
+      nmin = fmin(nmin, nw0);
       wk.imf[i] = nw0;
       if (i + 1 < N) {
         const double nw1 = old1 - 0.5 * (emax1 + emin1);
@@ -1054,44 +1072,51 @@ __device__ __forceinline__ SiftStats envelope_update(const Work& wk, const Tile&
         const double r1 = d1 / nw1;
         ln = ln + nw1 * nw1;
         ld = ld + d1 * d1;
-        lo = lo + old1 * old1;
         ls = ls + r1 * r1;
-        omax = fmax(omax, old1);
-        omin = fmin(omin, old1);
+        nmax = fmax(nmax, nw1);
+        nmin = fmin(nmin, nw1);
         wk.imf[i + 1] = nw1;
       }
     }
-    ln = wave_tree_sum(ln);
-    ld = wave_tree_sum(ld);
-    lo = wave_tree_sum(lo);
-    ls = wave_tree_sum(ls);
-    if (lane == 0) {
+    ln = wave_tree_sum_last(ln);
+    ld = wave_tree_sum_last(ld);
+    ls = wave_tree_sum_last(ls);
+    if (lane == 63) {
       wk.bsum[c] = ln;
       wk.bsum[wk.nbcap + c] = ld;
-      wk.bsum[2 * wk.nbcap + c] = lo;
-      wk.bsum[3 * wk.nbcap + c] = ls;
+      wk.bsum[2 * wk.nbcap + c] = ls;
     }
   }
-  omax = wave_max(omax);
-  omin = wave_min(omin);
+  nmax = wave_max(nmax);
+  nmin = wave_min(nmin);
   if (lane == 0) {
-    sh.red[w][4] = omax;
-    sh.red[w][5] = omin;
+    sh.red[w][2] = nmax;
+    sh.red[w][3] = nmin;
   }
   __syncthreads();
   SiftStats st;
   st.sq_new = blocks_tree(wk.bsum, nblk2, lane);
   st.sq_diff = blocks_tree(wk.bsum + wk.nbcap, nblk2, lane);
-  st.sq_old = blocks_tree(wk.bsum + 2 * wk.nbcap, nblk2, lane);
-  st.std_sum = blocks_tree(wk.bsum + 3 * wk.nbcap, nblk2, lane);
-  double gmx = sh.red[0][4], gmn = sh.red[0][5];
+  st.std_sum = blocks_tree(wk.bsum + 2 * wk.nbcap, nblk2, lane);
+  double gnx = sh.red[0][2], gnn = sh.red[0][3];
 #pragma unroll
   for (int q = 1; q < NW; ++q) {
-    gmx = fmax(gmx, sh.red[q][4]);
-    gmn = fmin(gmn, sh.red[q][5]);
+    gnx = fmax(gnx, sh.red[q][2]);
+    gnn = fmin(gnn, sh.red[q][3]);
   }
-  st.old_max = gmx;
-  st.old_min = gmn;
+  {
+    const double* c = sh.carry[(n - 1) & 1];
+    st.sq_old = c[0];
+    st.old_max = c[1];
+    st.old_min = c[2];
+  }
+  // slot n&1 was last read by sift n-1, which every thread has finished (barriers since)
+  if (tid == 0) {
+    double* c = sh.carry[n & 1];
+    c[0] = st.sq_new;
+    c[1] = gnx;
+    c[2] = gnn;
+  }
   return st;
 }
 
@@ -1189,8 +1214,42 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
   int sifts = 0, peak = 0, knot_sum = 0, flags = 0;
 
   while (!finished && imf_count < P.max_imfs - 1) {
-    for (int i = tid; i < N; i += WG) wk.imf[i] = io.S[i] - (have_acc ? acc[i] : 0.0);
-    __syncthreads();
+    // imf = S - sum of accepted rows, with its sum of squares and range for the first sift
+    {
+      double omax = -INFINITY, omin = INFINITY;
+      for (int blk = w; blk < nblk; blk += NW) {
+        const int i = blk * 64 + lane;
+        double lo = 0.0;
+        if (i < N) {
+          const double v = io.S[i] - (have_acc ? acc[i] : 0.0);
+          wk.imf[i] = v;
+          lo = v * v;
+          omax = fmax(omax, v);
+          omin = fmin(omin, v);
+        }
+        lo = wave_tree_sum_last(lo);
+        if (lane == 63) wk.bsum[blk] = lo;
+      }
+      omax = wave_max(omax);
+      omin = wave_min(omin);
+      if (lane == 0) {
+        sh.red[w][2] = omax;
+        sh.red[w][3] = omin;
+      }
+      __syncthreads();
+      const double sq = blocks_tree(wk.bsum, nblk, lane);
+      double gmx = sh.red[0][2], gmn = sh.red[0][3];
+#pragma unroll
+      for (int q = 1; q < NW; ++q) {
+        gmx = fmax(gmx, sh.red[q][2]);
+        gmn = fmin(gmn, sh.red[q][3]);
+      }
+      if (tid == 0) {
+        sh.carry[0][0] = sq;
+        sh.carry[0][1] = gmx;
+        sh.carry[0][2] = gmn;
+      }
+    }
     DetectResult D = detect<WG>(wk, N, sh, tl, tg, P);
     overflow |= D.ovf;
     peak = max(peak, max(D.tm, D.tn));
@@ -1211,11 +1270,11 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
         SiftStats st;
         if (D.in_lds) {
           build_envelopes<WG>(tl, sh, D, N, P, wk);
-          st = envelope_update<WG>(wk, tl, sh, N, D, nb);
+          st = envelope_update<WG>(wk, tl, sh, N, D, nb, n);
         } else {
           flags |= 2;
           build_envelopes<WG>(tg, sh, D, N, P, wk);
-          st = envelope_update<WG>(wk, tg, sh, N, D, nb);
+          st = envelope_update<WG>(wk, tg, sh, N, D, nb, n);
         }
         const int bad = uni(sh.bad);
         ++sifts;
@@ -1258,8 +1317,8 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
         tmax = fmax(tmax, tmp);
         tmin = fmin(tmin, tmp);
       }
-      la = wave_tree_sum(la);
-      if (lane == 0) wk.bsum[blk] = la;
+      la = wave_tree_sum_last(la);
+      if (lane == 63) wk.bsum[blk] = la;
     }
     tmax = wave_max(tmax);
     tmin = wave_min(tmin);
