@@ -283,26 +283,24 @@ __device__ __forceinline__ void compact_knots(const Work& wk, int N, const Tile&
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nblk = ceil_div(N, 64);
   const int tb0 = pad, tb1 = 3 * pad + nm;
+  const uint64_t lt = lanemask_lt(lane);
   for (int blk = w; blk < nblk; blk += NW) {
     const int i = blk * 64 + lane;
+    // every load of the block is issued before the first use (one LDS round trip)
     const uint64_t bm = wk.mmax[blk], bn = wk.mmin[blk];
-    if ((bm >> lane) & 1ull) {
-      const int k = wk.pmax[blk] + __popcll(bm & lanemask_lt(lane));
-      if (k < nm) {
-        const double y = wk.imf[i];
-        t.X[tb0 + k] = i;
-        t.Y[tb0 + k] = y;
-        bad |= (y < 0.0);
-      }
+    const int pm = wk.pmax[blk], pn = wk.pmin[blk];
+    const double y = wk.imf[i];
+    const int km = pm + __popcll(bm & lt);
+    const int kn = pn + __popcll(bn & lt);
+    if (((bm >> lane) & 1ull) && km < nm) {
+      t.X[tb0 + km] = i;
+      t.Y[tb0 + km] = y;
+      bad |= (y < 0.0);
     }
-    if ((bn >> lane) & 1ull) {
-      const int k = wk.pmin[blk] + __popcll(bn & lanemask_lt(lane));
-      if (k < nn) {
-        const double y = wk.imf[i];
-        t.X[tb1 + k] = i;
-        t.Y[tb1 + k] = y;
-        bad |= (y > 0.0);
-      }
+    if (((bn >> lane) & 1ull) && kn < nn) {
+      t.X[tb1 + kn] = i;
+      t.Y[tb1 + kn] = y;
+      bad |= (y > 0.0);
     }
   }
 }
@@ -321,24 +319,21 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
   int nz = 0, plat = 0;
   for (int blk = w; blk < nblk; blk += NW) {
     const int i = blk * 64 + lane;
-    bool is_max = false, is_min = false, zc = false, zr = false, pl = false;
-    if (i < N) {
-      const double x = imf[i];
-      const double prev = (i >= 1) ? imf[i - 1] : 0.0;
-      if (i <= N - 2) {
-        const double next = imf[i + 1];
-        const double d2 = next - x;
-        pl = (d2 == 0.0);
-        zc = (x * next < 0.0);
-        if (i >= 1) {
-          const double d1 = x - prev;
-          const double p = d1 * d2;
-          is_max = (p < 0.0) && (d1 > 0.0);
-          is_min = (p < 0.0) && (d1 < 0.0);
-        }
-      }
-      zr = (x == 0.0 && (i == 0 || prev != 0.0));
-    }
+    // all three loads stay inside the work area (i + 1 <= 64 * nbcap lands in the block
+    // sums that follow the IMF); values outside the trace are masked by the predicates
+    const double x = imf[i];
+    const double prev = imf[i >= 1 ? i - 1 : 0];
+    const double next = imf[i + 1];
+    const bool in = i < N, inner = i <= N - 2;
+    const double d2 = next - x;
+    const double d1 = x - prev;
+    const double p = d1 * d2;
+    const bool ext = in && inner && i >= 1 && (p < 0.0);
+    const bool is_max = ext && (d1 > 0.0);
+    const bool is_min = ext && (d1 < 0.0);
+    const bool pl = in && inner && (d2 == 0.0);
+    const bool zc = in && inner && (x * next < 0.0);
+    const bool zr = in && (x == 0.0) && (i == 0 || prev != 0.0);
     const uint64_t bm = __ballot(is_max), bn = __ballot(is_min);
     nz += __popcll(__ballot(zc)) + __popcll(__ballot(zr));
     plat |= (__ballot(pl) != 0ull);
