@@ -124,20 +124,6 @@ __device__ __forceinline__ int round_half_even_mid(int s) {
   return (f & 1) ? f + 1 : f;
 }
 
-// ---- wave helpers ---------------------------------------------------------------------
-__device__ __forceinline__ unsigned wave_incl_scan(unsigned v, int lane) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    unsigned t = __shfl_up(v, d, 64);
-    if (lane >= d) v += t;
-  }
-  return v;
-}
-__device__ __forceinline__ int wave_sum_int(int v) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) v += __shfl_xor(v, d, 64);
-  return v;
-}
 // Cross-lane moves for the butterflies below (all 64 lanes active).  DPP covers the
 // levels inside 16-lane rows: quad permutations for lane^1 and lane^2, then row mirrors
 // (lane i <- 7-i, lane i <- 15-i), which pair each lane with the other half of its group;
@@ -149,6 +135,9 @@ constexpr int kDppHalfMirror = 0x141; // row_half_mirror
 constexpr int kDppMirror = 0x140;     // row_mirror
 constexpr int kDppBcast15 = 0x142;    // row_bcast:15 (lane 15 of each row -> the next row)
 constexpr int kDppBcast31 = 0x143;    // row_bcast:31 (lane 31 -> rows 2 and 3)
+constexpr int kDppRowShr = 0x110;     // + n: row_shr:n (lane i <- lane i-n of its row)
+constexpr int kDppWaveShl1 = 0x130;   // wave_shl:1 (lane i <- lane i+1)
+constexpr int kDppWaveShr1 = 0x138;   // wave_shr:1 (lane i <- lane i-1)
 
 // Lanes of rows outside ROWS keep 0 (the `old` operand).
 template <int CTRL, int ROWS = 0xF>
@@ -193,6 +182,21 @@ __device__ __forceinline__ double wave_tree_sum_last(double v) {
   v += dpp_d<kDppMirror>(v);
   v += dpp_d<kDppBcast15, 0xA>(v);
   v += dpp_d<kDppBcast31, 0x8>(v);
+  return v;
+}
+// Inclusive prefix sum over the 64 lanes: row-local shifts by 1, 2, 4 and 8 (a lane whose
+// source lies outside its 16-lane row adds 0), then the row totals by broadcast.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ unsigned dpp_u(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, false);
+}
+__device__ __forceinline__ unsigned wave_incl_scan(unsigned v) {
+  v += dpp_u<kDppRowShr + 1>(v);
+  v += dpp_u<kDppRowShr + 2>(v);
+  v += dpp_u<kDppRowShr + 4>(v);
+  v += dpp_u<kDppRowShr + 8>(v);
+  v += dpp_u<kDppBcast15, 0xA>(v);
+  v += dpp_u<kDppBcast31, 0xC>(v);
   return v;
 }
 __device__ __forceinline__ double wave_max(double v) {
@@ -373,7 +377,7 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
       const int b = lane * per + k;
       if (b < nblk) loc += __popcll(wk.mmax[b]) | (unsigned(__popcll(wk.mmin[b])) << 16);
     }
-    const unsigned incl = wave_incl_scan(loc, lane);
+    const unsigned incl = wave_incl_scan(loc);
     unsigned run = incl - loc;
     for (int k = 0; k < per; ++k) {
       const int b = lane * per + k;
@@ -813,8 +817,8 @@ __device__ __forceinline__ void spline_system_wave(const Tile& t, int side, int 
     red = interior ? d - a * ue : d;
   }
   // odd-even step: eliminate x_s of this chunk and of the next from the e-row
-  const double nsa = __shfl_down(rsa, 1), nsb = __shfl_down(rsb, 1);
-  const double nsc = __shfl_down(rsc, 1), nsd = __shfl_down(rsd, 1);
+  const double nsa = dpp_d<kDppWaveShl1>(rsa), nsb = dpp_d<kDppWaveShl1>(rsb);
+  const double nsc = dpp_d<kDppWaveShl1>(rsc), nsd = dpp_d<kDppWaveShl1>(rsd);
   const bool has_next = lane + 1 < K;
   const double k1 = rea / rsb;
   const double k2 = has_next ? rec / nsb : 0.0;
@@ -822,12 +826,28 @@ __device__ __forceinline__ void spline_system_wave(const Tile& t, int side, int 
   double B = reb - k1 * rsc - (has_next ? k2 * nsa : 0.0);
   double C = has_next ? -(k2 * nsc) : 0.0;
   double D = red - k1 * rsd - (has_next ? k2 * nsd : 0.0);
-  // parallel cyclic reduction over the chunk e-rows
+  // parallel cyclic reduction over the chunk e-rows (distance 1 by whole-wave DPP shifts)
   for (int dd = 1; dd < K; dd <<= 1) {
-    const double Au = __shfl_up(A, dd), Bu = __shfl_up(B, dd);
-    const double Cu = __shfl_up(C, dd), Du = __shfl_up(D, dd);
-    const double Ad = __shfl_down(A, dd), Bdn = __shfl_down(B, dd);
-    const double Cd = __shfl_down(C, dd), Dd = __shfl_down(D, dd);
+    double Au, Bu, Cu, Du, Ad, Bdn, Cd, Dd;
+    if (dd == 1) {
+      Au = dpp_d<kDppWaveShr1>(A);
+      Bu = dpp_d<kDppWaveShr1>(B);
+      Cu = dpp_d<kDppWaveShr1>(C);
+      Du = dpp_d<kDppWaveShr1>(D);
+      Ad = dpp_d<kDppWaveShl1>(A);
+      Bdn = dpp_d<kDppWaveShl1>(B);
+      Cd = dpp_d<kDppWaveShl1>(C);
+      Dd = dpp_d<kDppWaveShl1>(D);
+    } else {
+      Au = __shfl_up(A, dd);
+      Bu = __shfl_up(B, dd);
+      Cu = __shfl_up(C, dd);
+      Du = __shfl_up(D, dd);
+      Ad = __shfl_down(A, dd);
+      Bdn = __shfl_down(B, dd);
+      Cd = __shfl_down(C, dd);
+      Dd = __shfl_down(D, dd);
+    }
     const bool hu = lane >= dd, hd = lane + dd < 64;
     const double q1 = hu ? A / Bu : 0.0;
     const double q2 = hd ? C / Bdn : 0.0;
@@ -841,7 +861,7 @@ __device__ __forceinline__ void spline_system_wave(const Tile& t, int side, int 
     D = nD;
   }
   const double xe = D / B;
-  const double xprev = __shfl_up(xe, 1);
+  const double xprev = dpp_d<kDppWaveShr1>(xe);
   if (act) {
     const double xs = (rsd - (lane > 0 ? rsa * xprev : 0.0) - rsc * xe) / rsb;
     t.M[k0 + 1 + s] = xs;
