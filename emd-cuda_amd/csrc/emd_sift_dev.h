@@ -163,6 +163,14 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The same for a knot tile that may live in global memory (the workgroup's global tile):
+// workgroup-scope fences also order the wave's global stores before its later loads.
+__device__ __forceinline__ void tile_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // Perfect binary tree over the 64 lanes (lane order); every lane gets the same bits:
 // level d adds the partner at lane^d, i.e. ((x0+x1)+(x2+x3))+... (fp addition commutes,
 // so both partners form bit-identical sums).
@@ -685,14 +693,23 @@ __device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int
 // global region).  The solution goes to M by knot slot; the end second derivatives are
 // then recovered and 1/h is stored per interval.
 // =======================================================================================
+// Per interval k (knots k, k+1) of a not-a-knot side: H[k] = 1/h and, until the solve
+// overwrites it, M[k] = the chord slope (y_{k+1} - y_k) / h as (y_{k+1} - y_k) * (1/h).
+__device__ __forceinline__ void interval_setup(const Tile& t, int k) {
+  const double ih = 1.0 / double(t.X[k + 1] - t.X[k]);
+  t.H[k] = ih;
+  t.M[k] = (t.Y[k + 1] - t.Y[k]) * ih;
+}
+
 // Row j of the system of a side whose knots start at slot k0: a x_{j-1} + b x_j + c x_{j+1} = d.
+// Reads the chord slopes the interval setup left in M.
 __device__ __forceinline__ void nak_row(const Tile& t, int k0, int m, int j, double& a, double& b,
                                         double& c, double& d) {
   const int i = k0 + j + 1;
   const double xa = t.X[i - 1], xb = t.X[i], xc = t.X[i + 1];
-  const double ya = t.Y[i - 1], yb = t.Y[i], yc = t.Y[i + 1];
+  const double sl = t.M[i - 1], sr = t.M[i];
   const double hl = xb - xa, hr = xc - xb;
-  const double rhs = 6.0 * ((yc - yb) / hr - (yb - ya) / hl);
+  const double rhs = 6.0 * (sr - sl);
   if (j == 0) {
     a = 0.0;
     b = hl + 2.0 * hr;
@@ -822,45 +839,43 @@ __device__ __forceinline__ void spline_system_wave(const Tile& t, int side, int 
   const bool has_next = lane + 1 < K;
   const double k1 = rea / rsb;
   const double k2 = has_next ? rec / nsb : 0.0;
-  double A = -(k1 * rsa);
-  double B = reb - k1 * rsc - (has_next ? k2 * nsa : 0.0);
-  double C = has_next ? -(k2 * nsc) : 0.0;
-  double D = red - k1 * rsd - (has_next ? k2 * nsd : 0.0);
-  // parallel cyclic reduction over the chunk e-rows (distance 1 by whole-wave DPP shifts)
+  const double A = -(k1 * rsa);
+  const double B = reb - k1 * rsc - (has_next ? k2 * nsa : 0.0);
+  const double C = has_next ? -(k2 * nsc) : 0.0;
+  const double D = red - k1 * rsd - (has_next ? k2 * nsd : 0.0);
+  // parallel cyclic reduction over the chunk e-rows, each row scaled to a unit diagonal
+  // (alpha x_{l-dd} + x_l + gamma x_{l+dd} = delta), so a level exchanges three values
+  // each way and divides once; distance 1 by whole-wave DPP shifts
+  double ib = 1.0 / B;
+  double al = A * ib, ga = C * ib, de = D * ib;
   for (int dd = 1; dd < K; dd <<= 1) {
-    double Au, Bu, Cu, Du, Ad, Bdn, Cd, Dd;
+    double au, gu, du, ad, gd, ddn;
     if (dd == 1) {
-      Au = dpp_d<kDppWaveShr1>(A);
-      Bu = dpp_d<kDppWaveShr1>(B);
-      Cu = dpp_d<kDppWaveShr1>(C);
-      Du = dpp_d<kDppWaveShr1>(D);
-      Ad = dpp_d<kDppWaveShl1>(A);
-      Bdn = dpp_d<kDppWaveShl1>(B);
-      Cd = dpp_d<kDppWaveShl1>(C);
-      Dd = dpp_d<kDppWaveShl1>(D);
+      au = dpp_d<kDppWaveShr1>(al);
+      gu = dpp_d<kDppWaveShr1>(ga);
+      du = dpp_d<kDppWaveShr1>(de);
+      ad = dpp_d<kDppWaveShl1>(al);
+      gd = dpp_d<kDppWaveShl1>(ga);
+      ddn = dpp_d<kDppWaveShl1>(de);
     } else {
-      Au = __shfl_up(A, dd);
-      Bu = __shfl_up(B, dd);
-      Cu = __shfl_up(C, dd);
-      Du = __shfl_up(D, dd);
-      Ad = __shfl_down(A, dd);
-      Bdn = __shfl_down(B, dd);
-      Cd = __shfl_down(C, dd);
-      Dd = __shfl_down(D, dd);
+      au = __shfl_up(al, dd);
+      gu = __shfl_up(ga, dd);
+      du = __shfl_up(de, dd);
+      ad = __shfl_down(al, dd);
+      gd = __shfl_down(ga, dd);
+      ddn = __shfl_down(de, dd);
     }
     const bool hu = lane >= dd, hd = lane + dd < 64;
-    const double q1 = hu ? A / Bu : 0.0;
-    const double q2 = hd ? C / Bdn : 0.0;
-    const double nA = hu ? -(q1 * Au) : 0.0;
-    const double nC = hd ? -(q2 * Cd) : 0.0;
-    const double nB = B - (hu ? q1 * Cu : 0.0) - (hd ? q2 * Ad : 0.0);
-    const double nD = D - (hu ? q1 * Du : 0.0) - (hd ? q2 * Dd : 0.0);
-    A = nA;
-    B = nB;
-    C = nC;
-    D = nD;
+    const double nA = hu ? -(al * au) : 0.0;
+    const double nC = hd ? -(ga * gd) : 0.0;
+    const double nB = 1.0 - (hu ? al * gu : 0.0) - (hd ? ga * ad : 0.0);
+    const double nD = de - (hu ? al * du : 0.0) - (hd ? ga * ddn : 0.0);
+    ib = 1.0 / nB;
+    al = nA * ib;
+    ga = nC * ib;
+    de = nD * ib;
   }
-  const double xe = D / B;
+  const double xe = de;
   const double xprev = dpp_d<kDppWaveShr1>(xe);
   if (act) {
     const double xs = (rsd - (lane > 0 ? rsa * xprev : 0.0) - rsc * xe) / rsb;
@@ -879,8 +894,33 @@ __device__ __forceinline__ void spline_system_wave(const Tile& t, int side, int 
   }
 }
 
+// Not-a-knot solve of one side by the calling wave (every lane calls): interval setup of
+// the intervals that touch mirrored knots (the true-knot intervals were set up while the
+// mirror ran), the solve, then M_0 and M_{n-1} from the not-a-knot relations.
+__device__ __forceinline__ void solve_side(const Tile& t, int side, int k0, int nk, int tb,
+                                           int n, int lane) {
+  const int nl = tb - k0;  // left mirrored knots
+  const int nmir = nk - n;  // intervals with a mirrored end: nl on the left, the rest right
+  if (lane < nmir) interval_setup(t, lane < nl ? k0 + lane : tb + n - 1 + (lane - nl));
+  tile_sync();
+  spline_system_wave(t, side, k0, nk - 2, lane);
+  tile_sync();
+  if (lane == 0) {
+    // all loads first, so they are in flight together
+    const double x0 = t.X[k0], x1 = t.X[k0 + 1], x2 = t.X[k0 + 2];
+    const double M1 = t.M[k0 + 1], M2 = t.M[k0 + 2];
+    const double xa = t.X[k0 + nk - 3], xb = t.X[k0 + nk - 2], xc = t.X[k0 + nk - 1];
+    const double Mb = t.M[k0 + nk - 2], Ma = t.M[k0 + nk - 3];
+    const double h0 = x1 - x0, h1 = x2 - x1;
+    const double hnm3 = xb - xa, hnm2 = xc - xb;
+    t.M[k0] = ((h0 + h1) * M1 - h0 * M2) / h1;
+    t.M[k0 + nk - 1] = ((hnm3 + hnm2) * Mb - hnm2 * Ma) / hnm3;
+  }
+}
+
 template <int WG>
-__device__ __forceinline__ void spline_solve(const Tile& t, const SiftShared<WG>& sh) {
+__device__ __forceinline__ void spline_solve(const Tile& t, const SiftShared<WG>& sh,
+                                             const DetectResult& D, int nb) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int mtot = uni(sh.mtot);
   if (mtot > 0) {
@@ -890,27 +930,8 @@ __device__ __forceinline__ void spline_solve(const Tile& t, const SiftShared<WG>
     for (int side = 0; side < 2; ++side) {
       const int owner = WG >= 128 ? side : 0;
       if (wv == owner && uni(sh.kind[side]) == kNotAKnot)
-        spline_system_wave(t, side, uni(sh.k0[side]), uni(sh.nk[side]) - 2, lane);
-    }
-    __syncthreads();
-    const int side = (tid == 0) ? 0 : (tid == 1 ? 1 : -1);
-    if (side >= 0 && sh.kind[side] == kNotAKnot) {
-      const int n = sh.nk[side], k0 = sh.k0[side];
-      // all loads first, so they are in flight together
-      const double x0 = t.X[k0], x1 = t.X[k0 + 1], x2 = t.X[k0 + 2];
-      const double M1 = t.M[k0 + 1], M2 = t.M[k0 + 2];
-      const double xa = t.X[k0 + n - 3], xb = t.X[k0 + n - 2], xc = t.X[k0 + n - 1];
-      const double Mb = t.M[k0 + n - 2], Ma = t.M[k0 + n - 3];
-      const double h0 = x1 - x0, h1 = x2 - x1;
-      const double hnm3 = xb - xa, hnm2 = xc - xb;
-      t.M[k0] = ((h0 + h1) * M1 - h0 * M2) / h1;
-      t.M[k0 + n - 1] = ((hnm3 + hnm2) * Mb - hnm2 * Ma) / hnm3;
-    }
-    // 1/h per interval by knot slot
-    for (int sd = 0; sd < 2; ++sd) {
-      if (sh.kind[sd] != kNotAKnot) continue;
-      const int k0 = sh.k0[sd], n = sh.nk[sd];
-      for (int k = tid; k < n - 1; k += WG) t.H[k0 + k] = 1.0 / double(t.X[k0 + k + 1] - t.X[k0 + k]);
+        solve_side(t, side, uni(sh.k0[side]), uni(sh.nk[side]), side ? 3 * nb + D.nm : nb,
+                   side ? D.nn : D.nm, lane);
     }
   }
   __syncthreads();
@@ -1152,6 +1173,7 @@ template <int WG>
 __device__ __forceinline__ void build_envelopes(const Tile& t, SiftShared<WG>& sh,
                                                 const DetectResult& D, int N,
                                                 const SiftParams& P, const Work& wk) {
+  constexpr int NW = WG / 64;
   if (threadIdx.x < 64) {
     int bad = sh.bad;
     int k0[2], nk[2], kind[2];
@@ -1170,8 +1192,16 @@ __device__ __forceinline__ void build_envelopes(const Tile& t, SiftShared<WG>& s
       sh.mtot = m0 + m1;
     }
   }
+  // interval setup between consecutive true knots of both sides (the mirror does not move
+  // them): waves 1.. while wave 0 mirrors; wave 0 itself when it is the only one
+  if (NW == 1 || threadIdx.x >= 64) {
+    const int nt0 = max(D.nm - 1, 0), nt = nt0 + max(D.nn - 1, 0);
+    const int tb0 = P.nbsym, tb1 = 3 * P.nbsym + D.nm;
+    for (int q = NW == 1 ? threadIdx.x : threadIdx.x - 64; q < nt; q += (NW == 1 ? 64 : WG - 64))
+      interval_setup(t, q < nt0 ? tb0 + q : tb1 + (q - nt0));
+  }
   __syncthreads();
-  spline_solve<WG>(t, sh);
+  spline_solve<WG>(t, sh, D, P.nbsym);
 }
 
 // =======================================================================================
