@@ -117,6 +117,15 @@ __device__ __forceinline__ int next_pow2(int x) {
   while (p < x) p <<= 1;
   return p;
 }
+// threadIdx.x behind an empty asm: each phase recomputes its lane-derived values and
+// addresses (a few VALU) instead of keeping them live, and spilled, across the whole
+// decomposition.
+__device__ __forceinline__ int fresh_tid() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 __device__ __forceinline__ int round_half_even_mid(int s) {
   // np.round(s / 2.0) for integer s >= 0
   int f = s >> 1;
@@ -292,7 +301,7 @@ template <int WG>
 __device__ __forceinline__ void compact_knots(const Work& wk, int N, const Tile& t, int nm,
                                               int nn, int pad, int& bad) {
   constexpr int NW = WG / 64;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int tid = fresh_tid(), lane = tid & 63, w = tid >> 6;
   const int nblk = ceil_div(N, 64);
   const int tb0 = pad, tb1 = 3 * pad + nm;
   const uint64_t lt = lanemask_lt(lane);
@@ -322,7 +331,7 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
                                                const Tile& tl, const Tile& tg,
                                                const SiftParams& P) {
   constexpr int NW = WG / 64;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = fresh_tid(), lane = tid & 63, w = tid >> 6;
   const int pad = P.nbsym;
   const int nblk = ceil_div(N, 64);
   const double* imf = wk.imf;
@@ -540,7 +549,7 @@ __device__ __forceinline__ void finish_lists(const Tile& t, const KnotEnds& c, i
                                              int rsym, int* k0, int* nk, int* kind, int& bad,
                                              double (*p3)[4]) {
   constexpr int kR = kMaxNbsym + 1;
-  const int lane = threadIdx.x & 63;
+  const int lane = fresh_tid() & 63;
   int nl[2], rs[2];
   side_counts(c, 0, nm, tb0, lmax, lsym, rmax, rsym, nl[0], rs[0], k0[0], nk[0], kind[0]);
   side_counts(c, 1, nn, tb1, lmin, lsym, rmin, rsym, nl[1], rs[1], k0[1], nk[1], kind[1]);
@@ -605,7 +614,7 @@ __device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int
                                        double SN, int* k0, int* nk, int* kind, int& bad,
                                        double (*p3)[4]) {
   const int tb0 = nb, tb1 = 3 * nb + nm;
-  const KnotEnds c = load_ends(t, tb0, tb1, nm, nn, nb + 1, threadIdx.x & 63, N, S0, SN);
+  const KnotEnds c = load_ends(t, tb0, tb1, nm, nn, nb + 1, fresh_tid() & 63, N, S0, SN);
   const int Xm0 = ends_x(c, 0, 0), XmL = ends_x(c, 0, nm - 1);
   const int Xn0 = ends_x(c, 1, 0), XnL = ends_x(c, 1, nn - 1);
   const double Ym0 = ends_y(c, 0, 0), YmL = ends_y(c, 0, nm - 1);
@@ -921,7 +930,7 @@ __device__ __forceinline__ void solve_side(const Tile& t, int side, int k0, int 
 template <int WG>
 __device__ __forceinline__ void spline_solve(const Tile& t, const SiftShared<WG>& sh,
                                              const DetectResult& D, int nb) {
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = fresh_tid(), lane = tid & 63, wv = tid >> 6;
   const int mtot = uni(sh.mtot);
   if (mtot > 0) {
     // wave 0 solves the maxima envelope and wave 1 the minima (wave 0 both when WG = 64);
@@ -991,22 +1000,26 @@ __device__ __forceinline__ double envelope_at(const Tile& t, const SiftShared<WG
 
 // Perfect binary tree over nblk block partials (zero-padded to a power of two), computed
 // identically by every lane of every wave.
-__device__ __forceinline__ double blocks_tree(const double* v, int nblk, int lane) {
+__device__ __forceinline__ double blocks_tree(const double* v, int nblk) {
+  const int lane = fresh_tid() & 63;
   int P = 1;
   while (P < nblk) P <<= 1;
   const int per = P > 64 ? P / 64 : 1;  // leaves per lane (1, 2 or 4)
+  const int last = nblk - 1;            // loads clamp to the last leaf; selects zero-pad
   double a = 0.0;
   if (per == 1) {
-    a = lane < nblk ? v[lane] : 0.0;
+    const double x0 = v[min(lane, last)];
+    a = lane < nblk ? x0 : 0.0;
   } else if (per == 2) {
     const int b = 2 * lane;
-    const double x0 = b < nblk ? v[b] : 0.0, x1 = b + 1 < nblk ? v[b + 1] : 0.0;
-    a = x0 + x1;
+    const double x0 = v[min(b, last)], x1 = v[min(b + 1, last)];
+    a = (b < nblk ? x0 : 0.0) + (b + 1 < nblk ? x1 : 0.0);
   } else {
     const int b = 4 * lane;
-    const double x0 = b < nblk ? v[b] : 0.0, x1 = b + 1 < nblk ? v[b + 1] : 0.0;
-    const double x2 = b + 2 < nblk ? v[b + 2] : 0.0, x3 = b + 3 < nblk ? v[b + 3] : 0.0;
-    a = (x0 + x1) + (x2 + x3);
+    const double x0 = v[min(b, last)], x1 = v[min(b + 1, last)];
+    const double x2 = v[min(b + 2, last)], x3 = v[min(b + 3, last)];
+    a = ((b < nblk ? x0 : 0.0) + (b + 1 < nblk ? x1 : 0.0)) +
+        ((b + 2 < nblk ? x2 : 0.0) + (b + 3 < nblk ? x3 : 0.0));
   }
   // lanes past the leaves hold zeros: the 64-leaf tree equals the tree over P leaves
   return wave_tree_sum(a);
@@ -1068,7 +1081,7 @@ __device__ __forceinline__ SiftStats envelope_update(const Work& wk, const Tile&
                                                      SiftShared<WG>& sh, int N,
                                                      const DetectResult& D, int nb, int n) {
   constexpr int NW = WG / 64;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = fresh_tid(), lane = tid & 63, w = tid >> 6;
   const int nblk2 = ceil_div(N, 128);
   const int tb0 = nb, tb1 = 3 * nb + D.nm;
   const int kind0 = uni(sh.kind[0]), kind1 = uni(sh.kind[1]);
@@ -1131,9 +1144,9 @@ __device__ __forceinline__ SiftStats envelope_update(const Work& wk, const Tile&
   }
   __syncthreads();
   SiftStats st;
-  st.sq_new = blocks_tree(wk.bsum, nblk2, lane);
-  st.sq_diff = blocks_tree(wk.bsum + wk.nbcap, nblk2, lane);
-  st.std_sum = blocks_tree(wk.bsum + 2 * wk.nbcap, nblk2, lane);
+  st.sq_new = blocks_tree(wk.bsum, nblk2);
+  st.sq_diff = blocks_tree(wk.bsum + wk.nbcap, nblk2);
+  st.std_sum = blocks_tree(wk.bsum + 2 * wk.nbcap, nblk2);
   double gnx = sh.red[0][2], gnn = sh.red[0][3];
 #pragma unroll
   for (int q = 1; q < NW; ++q) {
@@ -1225,7 +1238,7 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
                                           const Work& wk, const Tile& tl, const Tile& tg,
                                           const SiftParams& P) {
   constexpr int NW = WG / 64;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = fresh_tid(), lane = tid & 63, w = tid >> 6;
   const int N = io.N;
   const int nblk = ceil_div(N, 64);
   const int nb = P.nbsym;
@@ -1282,7 +1295,7 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
         sh.red[w][3] = omin;
       }
       __syncthreads();
-      const double sq = blocks_tree(wk.bsum, nblk, lane);
+      const double sq = blocks_tree(wk.bsum, nblk);
       double gmx = sh.red[0][2], gmn = sh.red[0][3];
 #pragma unroll
       for (int q = 1; q < NW; ++q) {
@@ -1373,7 +1386,7 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
     }
     __syncthreads();
     {
-      const double gsa = blocks_tree(wk.bsum, nblk, lane);
+      const double gsa = blocks_tree(wk.bsum, nblk);
       double gmx = sh.red[0][0], gmn = sh.red[0][1];
 #pragma unroll
       for (int q = 1; q < NW; ++q) {
@@ -1520,7 +1533,7 @@ template <int WG>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void emd_envelope_kernel(EnvBatch e, SiftParams P) {
   extern __shared__ double dyn_lds[];
   __shared__ SiftShared<WG> sh;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = fresh_tid(), lane = tid & 63, w = tid >> 6;
   constexpr int NW = WG / 64;
   const WgMem m = workgroup_mem(dyn_lds, e.spill, e.spill_stride, P);
   const int bidx = blockIdx.x;
