@@ -1232,6 +1232,72 @@ struct SignalIO {
   int* stats;       // 4 ints or null
 };
 
+constexpr int kPrefetch = 4;  // 64-sample blocks per wave whose global loads are batched
+
+// Start of an IMF: imf = S - acc (acc = the running sum of accepted rows, first updated
+// with the IMF in LDS when `add`), written to LDS with its sum of squares and range for
+// the first sift's imf_old (sh.carry[0]).
+template <int WG>
+__device__ __forceinline__ void start_imf(const SignalIO& io, SiftShared<WG>& sh, const Work& wk,
+                                          double* acc, bool have_acc, bool add) {
+  constexpr int NW = WG / 64;
+  const int tid = fresh_tid(), lane = tid & 63, w = tid >> 6;
+  const int N = io.N, nblk = ceil_div(N, 64);
+  // block sums in the fourth row of the work area's sums: the end-condition reduction
+  // that precedes this pass may still be reading the first
+  double* bs = wk.bsum + 3 * wk.nbcap;
+  double omax = -INFINITY, omin = INFINITY;
+  for (int blk0 = w; blk0 < nblk; blk0 += kPrefetch * NW) {
+    double sv[kPrefetch], av[kPrefetch];
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+      const int i = (blk0 + u * NW) * 64 + lane;
+      sv[u] = i < N ? io.S[i] : 0.0;
+      av[u] = (i < N && have_acc) ? acc[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+      const int blk = blk0 + u * NW;
+      if (blk >= nblk) break;
+      const int i = blk * 64 + lane;
+      double lo = 0.0;
+      if (i < N) {
+        double a = av[u];
+        if (add) {
+          a = a + wk.imf[i];
+          acc[i] = a;
+        }
+        const double v = sv[u] - a;
+        wk.imf[i] = v;
+        lo = v * v;
+        omax = fmax(omax, v);
+        omin = fmin(omin, v);
+      }
+      lo = wave_tree_sum_last(lo);
+      if (lane == 63) bs[blk] = lo;
+    }
+  }
+  omax = wave_max(omax);
+  omin = wave_min(omin);
+  if (lane == 0) {
+    sh.red[w][2] = omax;
+    sh.red[w][3] = omin;
+  }
+  __syncthreads();
+  const double sq = blocks_tree(bs, nblk);
+  double gmx = sh.red[0][2], gmn = sh.red[0][3];
+#pragma unroll
+  for (int q = 1; q < NW; ++q) {
+    gmx = fmax(gmx, sh.red[q][2]);
+    gmn = fmin(gmn, sh.red[q][3]);
+  }
+  if (tid == 0) {
+    sh.carry[0][0] = sq;
+    sh.carry[0][1] = gmx;
+    sh.carry[0][2] = gmn;
+  }
+}
+
 // tl: the launch's LDS knot tile; tg: this workgroup's global tile (holds any trace).
 template <int WG>
 __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh,
@@ -1272,42 +1338,7 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
   int sifts = 0, peak = 0, knot_sum = 0, flags = 0;
 
   while (!finished && imf_count < P.max_imfs - 1) {
-    // imf = S - sum of accepted rows, with its sum of squares and range for the first sift
-    {
-      double omax = -INFINITY, omin = INFINITY;
-      for (int blk = w; blk < nblk; blk += NW) {
-        const int i = blk * 64 + lane;
-        double lo = 0.0;
-        if (i < N) {
-          const double v = io.S[i] - (have_acc ? acc[i] : 0.0);
-          wk.imf[i] = v;
-          lo = v * v;
-          omax = fmax(omax, v);
-          omin = fmin(omin, v);
-        }
-        lo = wave_tree_sum_last(lo);
-        if (lane == 63) wk.bsum[blk] = lo;
-      }
-      omax = wave_max(omax);
-      omin = wave_min(omin);
-      if (lane == 0) {
-        sh.red[w][2] = omax;
-        sh.red[w][3] = omin;
-      }
-      __syncthreads();
-      const double sq = blocks_tree(wk.bsum, nblk);
-      double gmx = sh.red[0][2], gmn = sh.red[0][3];
-#pragma unroll
-      for (int q = 1; q < NW; ++q) {
-        gmx = fmax(gmx, sh.red[q][2]);
-        gmn = fmin(gmn, sh.red[q][3]);
-      }
-      if (tid == 0) {
-        sh.carry[0][0] = sq;
-        sh.carry[0][1] = gmx;
-        sh.carry[0][2] = gmn;
-      }
-    }
+    if (imf_count == 0) start_imf<WG>(io, sh, wk, acc, false, false);
     DetectResult D = detect<WG>(wk, N, sh, tl, tg, P);
     overflow |= D.ovf;
     peak = max(peak, max(D.tm, D.tn));
@@ -1364,19 +1395,32 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
     // accept the IMF row, then PyEMD end_condition on S - (acc + imf)
     double* row = io.out + (int64_t)imf_count * N;
     double tmax = -INFINITY, tmin = INFINITY;
-    for (int blk = w; blk < nblk; blk += NW) {
-      const int i = blk * 64 + lane;
-      double la = 0.0;
-      if (i < N) {
-        const double v = wk.imf[i];
-        row[i] = v;
-        const double tmp = io.S[i] - ((have_acc ? acc[i] : 0.0) + v);
-        la = fabs(tmp);
-        tmax = fmax(tmax, tmp);
-        tmin = fmin(tmin, tmp);
+    for (int blk0 = w; blk0 < nblk; blk0 += kPrefetch * NW) {
+      // the global loads of kPrefetch blocks are issued before the first use
+      double sv[kPrefetch], av[kPrefetch];
+#pragma unroll
+      for (int u = 0; u < kPrefetch; ++u) {
+        const int i = (blk0 + u * NW) * 64 + lane;
+        sv[u] = i < N ? io.S[i] : 0.0;
+        av[u] = (i < N && have_acc) ? acc[i] : 0.0;
       }
-      la = wave_tree_sum_last(la);
-      if (lane == 63) wk.bsum[blk] = la;
+#pragma unroll
+      for (int u = 0; u < kPrefetch; ++u) {
+        const int blk = blk0 + u * NW;
+        if (blk >= nblk) break;
+        const int i = blk * 64 + lane;
+        double la = 0.0;
+        if (i < N) {
+          const double v = wk.imf[i];
+          row[i] = v;
+          const double tmp = sv[u] - (av[u] + v);
+          la = fabs(tmp);
+          tmax = fmax(tmax, tmp);
+          tmin = fmin(tmin, tmp);
+        }
+        la = wave_tree_sum_last(la);
+        if (lane == 63) wk.bsum[blk] = la;
+      }
     }
     tmax = wave_max(tmax);
     tmin = wave_min(tmin);
@@ -1402,11 +1446,13 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
     const bool leaving = finished || capped;
     if (leaving && extNo_last <= 2) {
       --imf_count;  // the last row was a trend (PyEMD drops it); acc excludes it
+      __syncthreads();
     } else {
-      for (int i = tid; i < N; i += WG) acc[i] = (have_acc ? acc[i] : 0.0) + wk.imf[i];
+      // acc += imf, and the next IMF starts from S - acc (the same value PyEMD's
+      // S - sum(IMF) gives, rounding included)
+      start_imf<WG>(io, sh, wk, acc, have_acc, true);
       have_acc = true;
     }
-    __syncthreads();  // acc/bsum/red settled before the next IMF (or the residue) uses them
     if (leaving) break;  // PyEMD drops the trend row after leaving its loop
   }
 
