@@ -126,6 +126,16 @@ __device__ __forceinline__ int fresh_tid() {
   return t;
 }
 
+// 1/q by the hardware reciprocal and two Newton steps (within an ulp or two; only the
+// std-test sum uses it)
+__device__ __forceinline__ double recip(double q) {
+  double y = __builtin_amdgcn_rcp(q);
+  double e = __builtin_fma(-q, y, 1.0);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-q, y, 1.0);
+  return __builtin_fma(y, e, y);
+}
+
 __device__ __forceinline__ int round_half_even_mid(int s) {
   // np.round(s / 2.0) for integer s >= 0
   int f = s >> 1;
@@ -1040,7 +1050,7 @@ __device__ __forceinline__ void envelope_pair(const Tile& t, const SiftShared<WG
   const int k0 = sh.k0[side], nk = sh.nk[side];
   const uint64_t m = side == 0 ? wk.mmax[b] : wk.mmin[b];
   const int pre = side == 0 ? wk.pmax[b] : wk.pmin[b];
-  const uint64_t le_mask = p == 63 ? ~uint64_t(0) : ((uint64_t(2) << p) - 1);
+  const uint64_t le_mask = (uint64_t(2) << p) - 1;  // bits 0..p (p is even, so <= 62)
   const int le0 = pre + __popcll(m & le_mask);
   const int le1 = le0 + (int)((m >> (p + 1)) & 1);  // p is even, so p + 1 <= 63
 #pragma unroll
@@ -1067,6 +1077,64 @@ __device__ __forceinline__ void envelope_pair(const Tile& t, const SiftShared<WG
   }
 }
 
+// One 128-sample block c of the envelope pass: both envelopes at samples i and i+1 of this
+// lane, the IMF update and the block's three stop-statistic sums (lane 63 stores them).
+// NAK: both sides are known to be not-a-knot.
+template <int WG, bool NAK>
+__device__ __forceinline__ void envelope_block(const Work& wk, const Tile& t,
+                                               const SiftShared<WG>& sh, int N, int c, int lane,
+                                               int tb0, int tb1, int kind0, int kind1,
+                                               double& nmax, double& nmin) {
+  const int i = c * 128 + 2 * lane;
+  double ln = 0.0, ld = 0.0, ls = 0.0;
+  if (i < N) {
+    const int b = 2 * c + (lane >> 5), p = (2 * lane) & 63;
+    double emax0, emax1, emin0, emin1;
+    if (NAK || kind0 == kNotAKnot) {
+      envelope_pair(t, sh, 0, wk, b, p, i, tb0, emax0, emax1);
+    } else {
+      emax0 = eval_small(t, sh.k0[0], kind0, sh.p3[0], i);
+      emax1 = eval_small(t, sh.k0[0], kind0, sh.p3[0], i + 1);
+    }
+    if (NAK || kind1 == kNotAKnot) {
+      envelope_pair(t, sh, 1, wk, b, p, i, tb1, emin0, emin1);
+    } else {
+      emin0 = eval_small(t, sh.k0[1], kind1, sh.p3[1], i);
+      emin1 = eval_small(t, sh.k0[1], kind1, sh.p3[1], i + 1);
+    }
+    // i + 1 < 64 * nbcap always (i is even and < N), so the pair read stays in the work area
+    const double old0 = wk.imf[i], old1 = wk.imf[i + 1];
+    const double nw0 = old0 - 0.5 * (emax0 + emin0);
+    const double d0 = nw0 - old0;
+    const double r0 = d0 * recip(nw0);
+    ln = nw0 * nw0;
+    ld = d0 * d0;
+    ls = r0 * r0;
+    nmax = fmax(nmax, nw0);
+    nmin = fmin(nmin, nw0);
+    wk.imf[i] = nw0;
+    if (i + 1 < N) {
+      const double nw1 = old1 - 0.5 * (emax1 + emin1);
+      const double d1 = nw1 - old1;
+      const double r1 = d1 * recip(nw1);
+      ln = ln + nw1 * nw1;
+      ld = ld + d1 * d1;
+      ls = ls + r1 * r1;
+      nmax = fmax(nmax, nw1);
+      nmin = fmin(nmin, nw1);
+      wk.imf[i + 1] = nw1;
+    }
+  }
+  ln = wave_tree_sum_last(ln);
+  ld = wave_tree_sum_last(ld);
+  ls = wave_tree_sum_last(ls);
+  if (lane == 63) {
+    wk.bsum[c] = ln;
+    wk.bsum[wk.nbcap + c] = ld;
+    wk.bsum[2 * wk.nbcap + c] = ls;
+  }
+}
+
 // One sift: evaluate both envelopes at every sample, subtract the mean, accumulate the
 // PyEMD stop statistics.  The IMF in LDS is updated in place.  Each lane takes two
 // adjacent samples of a 128-sample block; the pair is added first, then the 64-lane
@@ -1086,55 +1154,13 @@ __device__ __forceinline__ SiftStats envelope_update(const Work& wk, const Tile&
   const int tb0 = nb, tb1 = 3 * nb + D.nm;
   const int kind0 = uni(sh.kind[0]), kind1 = uni(sh.kind[1]);
   double nmax = -INFINITY, nmin = INFINITY;
-  for (int c = w; c < nblk2; c += NW) {
-    const int i = c * 128 + 2 * lane;
-    double ln = 0.0, ld = 0.0, ls = 0.0;
-    if (i < N) {
-      const int b = 2 * c + (lane >> 5), p = (2 * lane) & 63;
-      double emax0, emax1, emin0, emin1;
-      if (kind0 == kNotAKnot) {
-        envelope_pair(t, sh, 0, wk, b, p, i, tb0, emax0, emax1);
-      } else {
-        emax0 = eval_small(t, sh.k0[0], kind0, sh.p3[0], i);
-        emax1 = eval_small(t, sh.k0[0], kind0, sh.p3[0], i + 1);
-      }
-      if (kind1 == kNotAKnot) {
-        envelope_pair(t, sh, 1, wk, b, p, i, tb1, emin0, emin1);
-      } else {
-        emin0 = eval_small(t, sh.k0[1], kind1, sh.p3[1], i);
-        emin1 = eval_small(t, sh.k0[1], kind1, sh.p3[1], i + 1);
-      }
-      // i + 1 < 64 * nbcap always (i is even and < N), so the pair read stays in the work area
-      const double old0 = wk.imf[i], old1 = wk.imf[i + 1];
-      const double nw0 = old0 - 0.5 * (emax0 + emin0);
-      const double d0 = nw0 - old0;
-      const double r0 = d0 / nw0;
-      ln = nw0 * nw0;
-      ld = d0 * d0;
-      ls = r0 * r0;
-      nmax = fmax(nmax, nw0);
-      nmin = fmin(nmin, nw0);
-      wk.imf[i] = nw0;
-      if (i + 1 < N) {
-        const double nw1 = old1 - 0.5 * (emax1 + emin1);
-        const double d1 = nw1 - old1;
-        const double r1 = d1 / nw1;
-        ln = ln + nw1 * nw1;
-        ld = ld + d1 * d1;
-        ls = ls + r1 * r1;
-        nmax = fmax(nmax, nw1);
-        nmin = fmin(nmin, nw1);
-        wk.imf[i + 1] = nw1;
-      }
-    }
-    ln = wave_tree_sum_last(ln);
-    ld = wave_tree_sum_last(ld);
-    ls = wave_tree_sum_last(ls);
-    if (lane == 63) {
-      wk.bsum[c] = ln;
-      wk.bsum[wk.nbcap + c] = ld;
-      wk.bsum[2 * wk.nbcap + c] = ls;
-    }
+  // the common case (both envelopes not-a-knot) gets a loop without the small-kind paths
+  if (kind0 == kNotAKnot && kind1 == kNotAKnot) {
+    for (int c = w; c < nblk2; c += NW)
+      envelope_block<WG, true>(wk, t, sh, N, c, lane, tb0, tb1, kind0, kind1, nmax, nmin);
+  } else {
+    for (int c = w; c < nblk2; c += NW)
+      envelope_block<WG, false>(wk, t, sh, N, c, lane, tb0, tb1, kind0, kind1, nmax, nmin);
   }
   nmax = wave_max(nmax);
   nmin = wave_min(nmin);
