@@ -96,7 +96,10 @@ def train_cost_model(launcher, args):
     torch.cuda.synchronize()
     return cm, {"train_traces": int(args.train), "train_seed": 3, "epochs": args.epochs,
                 "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1],
-                "train_s": time.perf_counter() - t0}
+                "train_s": time.perf_counter() - t0,
+                "sift_rank_holdout": cm.sift_rank,
+                "queue_order": "predicted sifts x N" if cm.orders_by_sifts else
+                               "N, stable (predicted sifts do not rank held-out traces)"}
 
 
 def bench_ragged(args):
@@ -158,8 +161,11 @@ def bench_ragged(args):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"ragged mix, {B} traces, N log-uniform [256, 8192], seed 2",
                    "traces": B, "mean_N": float(lengths.mean())},
-        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+        # the compute roof: MI355X's fp64 peak (78.6 TFLOP/s, the same for its vector ALUs,
+        # which run the sift, as for its matrix cores, which it does not use)
+        "roofline": {"bound": "mfma", "roof": "fp64 arithmetic (VALU)", "achieved": achieved,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "flops_per_step": flops,
                      "note": "step time of all bucket launches (concurrent streams)"},
         "plain_traces_per_s": B / (res["plain"][1] * 1e-3),
@@ -458,8 +464,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": "seismic traces, 16384 x N=2001, seed 1", "traces": B, "N": N,
                    "path": head, "max_imfs": cfg.max_imfs, "max_extrema": cfg.max_extrema},
-        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+        # the compute roof: MI355X's fp64 peak (78.6 TFLOP/s, the same for its vector ALUs,
+        # which run the sift, as for its matrix cores, which it does not use)
+        "roofline": {"bound": "mfma", "roof": "fp64 arithmetic (VALU)", "achieved": achieved,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": measured_traffic() if head == "packed" and B == 16384 else None,
                      "flops_per_launch": flops,
                      "algorithmic_hbm_bytes": algorithmic_bytes(r["n_imfs"], lengths)},

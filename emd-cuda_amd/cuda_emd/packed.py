@@ -228,7 +228,8 @@ class PackedEMD:
     def _order(self, idx, pred):
         if pred is None:
             return idx
-        cost = np.maximum(pred[idx, 0], 1.0) * self.lengths[idx]
+        sifts = np.maximum(pred[idx, 0], 1.0) if self.cost_model.orders_by_sifts else 1.0
+        cost = sifts * self.lengths[idx]
         return idx[np.argsort(-cost, kind="stable")]
 
     def refresh_order(self) -> None:
