@@ -88,8 +88,9 @@ struct Shape {
 };
 
 int check_wg(int wg) {
-  if (wg != 64 && wg != 128 && wg != 256)
-    return fail(EMD_ERR_VALUE, "threads per workgroup must be 64, 128 or 256, got %d", wg);
+  if (wg != 64 && wg != 128 && wg != 256 && wg != 512 && wg != 1024)
+    return fail(EMD_ERR_VALUE, "threads per workgroup must be 64, 128, 256, 512 or 1024, got %d",
+                wg);
   return EMD_OK;
 }
 
@@ -182,8 +183,9 @@ int emd_create(const emd_config* cfg, emd_handle** out) {
     return fail(EMD_ERR_VALUE, "max_iteration must be >= 1, got %d", cfg->max_iteration);
   if (cfg->max_imfs < 1) return fail(EMD_ERR_VALUE, "max_imfs must be >= 1, got %d", cfg->max_imfs);
   const int tpb = cfg->threads_per_block;
-  if (tpb != 64 && tpb != 128 && tpb != 256)
-    return fail(EMD_ERR_VALUE, "threads_per_block must be 64, 128 or 256 on gfx950, got %d", tpb);
+  if (tpb != 64 && tpb != 128 && tpb != 256 && tpb != 512 && tpb != 1024)
+    return fail(EMD_ERR_VALUE,
+                "threads_per_block must be 64, 128, 256, 512 or 1024 on gfx950, got %d", tpb);
   if (cfg->tile_slots < 0) return fail(EMD_ERR_VALUE, "tile_slots must be >= 0");
   emd_handle* h = new emd_handle();
   h->cfg = *cfg;

@@ -8,6 +8,8 @@ __global__ void reset_counter_kernel(unsigned* q) { *q = 0u; }
 void* sift_instance_64(int which);
 void* sift_instance_128(int which);
 void* sift_instance_256(int which);
+void* sift_instance_512(int which);
+void* sift_instance_1024(int which);
 
 int64_t tile_bytes(int slots) {
   return (((int64_t)slots * kSlotBytes) + 255) / 256 * 256;
@@ -22,6 +24,8 @@ int static_lds_bytes(int wg) {
     case 64: return (int)sizeof(SiftShared<64>);
     case 128: return (int)sizeof(SiftShared<128>);
     case 256: return (int)sizeof(SiftShared<256>);
+    case 512: return (int)sizeof(SiftShared<512>);
+    case 1024: return (int)sizeof(SiftShared<1024>);
   }
   return 0;
 }
@@ -36,6 +40,8 @@ static void* instance(int which, int wg) {
     case 64: return sift_instance_64(which);
     case 128: return sift_instance_128(which);
     case 256: return sift_instance_256(which);
+    case 512: return sift_instance_512(which);
+    case 1024: return sift_instance_1024(which);
   }
   return nullptr;
 }
@@ -46,7 +52,7 @@ static void* envelope_kernel_ptr(int wg) { return instance(2, wg); }
 int prepare_kernels(int lds_per_workgroup) {
   // Allow each kernel the whole per-workgroup LDS once, so launches (possibly under
   // stream capture) never change function attributes.
-  for (int wg : {64, 128, 256}) {
+  for (int wg : {64, 128, 256, 512, 1024}) {
     const int dyn = lds_per_workgroup - static_lds_bytes(wg);
     for (int which = 0; which < 3; ++which) {
       hipError_t e = hipFuncSetAttribute(instance(which, wg),

@@ -240,17 +240,6 @@ __device__ __forceinline__ double wave_min(double v) {
   v = fmin(v, dpp_d<kDppMirror>(v));
   return fmin(fmin(lane_d(v, 0), lane_d(v, 16)), fmin(lane_d(v, 32), lane_d(v, 48)));
 }
-template <int NW>
-__device__ __forceinline__ double waves_tree(const double* r, int stride) {
-  if constexpr (NW == 1) {
-    return r[0];
-  } else if constexpr (NW == 2) {
-    return r[0] + r[stride];
-  } else {
-    return (r[0] + r[stride]) + (r[2 * stride] + r[3 * stride]);
-  }
-}
-
 // =======================================================================================
 // Extrema detection.  Semantics of PyEMD _find_extrema_simple: strict extrema where
 // d1*d2 < 0; plateau extrema at the half-to-even rounded midpoint of d==0 runs (first run

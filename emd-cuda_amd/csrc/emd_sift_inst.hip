@@ -1,5 +1,6 @@
 // emd_sift_inst.hip -- the sift kernels for one workgroup size.
-// Compiled once per size with -DEMD_WG=<64|128|256> so the instances build in parallel;
+// Compiled once per size with -DEMD_WG=<64|128|256|512|1024> so the instances build in
+// parallel;
 // emd_dispatch.hip selects among them at run time.
 #include "emd_sift_dev.h"
 

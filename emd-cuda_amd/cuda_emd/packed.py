@@ -7,7 +7,7 @@ stream, so the tail of one bucket overlaps the next.  Per bucket:
 
 * the cost model (when given) orders the queue longest-predicted-first (sifts x N) and
   proposes the LDS knot tile (a high quantile of predicted peak extrema);
-* the autotuner picks the workgroup size (64/128/256), the LDS tile (the capacity rule, the
+* the autotuner picks the workgroup size (64 to 1024 lanes), the LDS tile (the capacity rule, the
   model's proposal, or the largest tile that lets 2, 3 or 4 workgroups share a CU) and the
   resident workgroups per CU by timing the candidates on a sample of the bucket's traces;
   results are cached in a JSON file.
@@ -63,7 +63,7 @@ class Autotuner:
     """Times launch candidates per length bucket on a sample of that bucket's traces."""
 
     def __init__(self, config: EMDKernelConfig, cache_path: str | None = None,
-                 sample: int = 1024, threads=(64, 128, 256)):
+                 sample: int = 1024, threads=(64, 128, 256, 512, 1024)):
         self.config = config
         self.cache_path = cache_path
         self.sample = sample

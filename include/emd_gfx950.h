@@ -42,7 +42,7 @@ typedef struct emd_handle emd_handle;
 
 /* Launch shape for the ragged/persistent entry point (chosen by the autotuner). */
 typedef struct emd_launch {
-  int threads;     /* 64, 128 or 256 */
+  int threads;     /* 64, 128, 256, 512 or 1024 */
   int tile_slots;  /* LDS knot slots (both envelopes); 0 = as emd_config */
   int wgs_per_cu;  /* persistent kernel: resident workgroups per CU (0 = occupancy max) */
   int persistent;  /* 1 = persistent atomic-queue kernel, 0 = one workgroup per trace */

@@ -30,5 +30,5 @@ def test_host_only_queries():
     lib = _lib.load()
     assert lib.emd_version() >= 1
     assert lib.emd_lds_slot_bytes() == 28
-    for wg in (64, 128, 256):
+    for wg in (64, 128, 256, 512, 1024):
         assert 0 < lib.emd_static_lds_bytes(wg) < 4096

@@ -29,7 +29,7 @@ def _chirps(B=40):
     return synth.chirp_mix(B, 1024, seed=7)
 
 
-@pytest.mark.parametrize("tpb", [64, 128, 256])
+@pytest.mark.parametrize("tpb", [64, 128, 256, 512, 1024])
 def test_workgroup_size_invariance(tpb):
     x = _chirps()
     ref = _padded(x)
