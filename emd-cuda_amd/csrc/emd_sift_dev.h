@@ -1380,10 +1380,13 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
           build_envelopes<WG>(tg, sh, D, N, P, wk);
           st = envelope_update<WG>(wk, tg, sh, N, D, nb, n);
         }
+        // No barrier before the next detection: what this sift's last steps read (block
+        // sums, sh.red, sh.carry, sh.bad, sh.nk) is written next only after the
+        // detection's first barrier (sh.bad by thread 0 after its last one), and its
+        // own writes (masks, counts, knots) touch nothing those steps read.
         const int bad = uni(sh.bad);
         ++sifts;
         knot_sum += uni(sh.nk[0] + sh.nk[1]);
-        __syncthreads();  // everyone has read sh/bsum before detect reuses them
         D = detect<WG>(wk, N, sh, tl, tg, P);
         overflow |= D.ovf;
         peak = max(peak, max(D.tm, D.tn));
