@@ -1410,8 +1410,13 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
       }
     }
 
-    // accept the IMF row, then PyEMD end_condition on S - (acc + imf)
+    // accept the IMF row, then PyEMD end_condition on S - (acc + imf).  Unless this row may
+    // be a trend about to be dropped (extNo_last <= 2), the same pass makes acc += imf and
+    // starts the next IMF: S - (acc + imf) is its first imf, rounding included, and its
+    // sum of squares and range are the next first sift's imf_old statistics.
+    const bool fuse = extNo_last > 2;
     double* row = io.out + (int64_t)imf_count * N;
+    double* bs3 = wk.bsum + 3 * wk.nbcap;
     double tmax = -INFINITY, tmin = INFINITY;
     for (int blk0 = w; blk0 < nblk; blk0 += kPrefetch * NW) {
       // the global loads of kPrefetch blocks are issued before the first use
@@ -1427,17 +1432,27 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
         const int blk = blk0 + u * NW;
         if (blk >= nblk) break;
         const int i = blk * 64 + lane;
-        double la = 0.0;
+        double la = 0.0, lo = 0.0;
         if (i < N) {
           const double v = wk.imf[i];
           row[i] = v;
-          const double tmp = sv[u] - (av[u] + v);
+          const double an = av[u] + v;
+          const double tmp = sv[u] - an;
           la = fabs(tmp);
           tmax = fmax(tmax, tmp);
           tmin = fmin(tmin, tmp);
+          if (fuse) {
+            acc[i] = an;
+            wk.imf[i] = tmp;
+            lo = tmp * tmp;
+          }
         }
         la = wave_tree_sum_last(la);
-        if (lane == 63) wk.bsum[blk] = la;
+        if (fuse) lo = wave_tree_sum_last(lo);
+        if (lane == 63) {
+          wk.bsum[blk] = la;
+          if (fuse) bs3[blk] = lo;
+        }
       }
     }
     tmax = wave_max(tmax);
@@ -1457,6 +1472,14 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
       }
       const int done = ((gmx - gmn) < kRangeThr || gsa < kTotalPowerThr) ? 1 : 0;
       if (uni(done)) finished = true;
+      if (fuse && w == 0) {
+        const double sq = blocks_tree(bs3, nblk);
+        if (lane == 0) {
+          sh.carry[0][0] = sq;
+          sh.carry[0][1] = gmx;
+          sh.carry[0][2] = gmn;
+        }
+      }
     }
     ++imf_count;
     const bool capped = imf_count >= P.max_imfs - 1;
@@ -1465,11 +1488,13 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
     if (leaving && extNo_last <= 2) {
       --imf_count;  // the last row was a trend (PyEMD drops it); acc excludes it
       __syncthreads();
-    } else {
+    } else if (!fuse) {
       // acc += imf, and the next IMF starts from S - acc (the same value PyEMD's
       // S - sum(IMF) gives, rounding included)
       start_imf<WG>(io, sh, wk, acc, have_acc, true);
       have_acc = true;
+    } else {
+      have_acc = true;  // the accept pass did both
     }
     if (leaving) break;  // PyEMD drops the trend row after leaving its loop
   }
