@@ -135,3 +135,30 @@ def test_imf_cap_with_trend_drop(seed):
     o = _oracle(x, max_imfs=3)
     assert o[0].imf_cap_hit and 0 <= o[0].debug_extno[1] <= 2
     _assert_parity(x, _gpu_padded(x, EMDKernelConfig(max_imfs=3)), o, max_imfs=3)
+
+
+@pytest.mark.parametrize("threads", [256, 1024])
+def test_long_trace_parity(threads):
+    """Traces of 6000-12000 samples (up to four 64-sample blocks per lane in the block
+    scans, more than 64 block edges) through the persistent CSR path, with 256 and with
+    1024 lanes per trace."""
+    import torch
+    from cuda_emd import EMDKernelConfig, synth
+    rng = np.random.default_rng(21)
+    sig = [synth.chirp_mix(1, 6000, seed=12)[0],
+           synth.chirp_mix(1, 9000, seed=13)[0] + 0.02 * rng.standard_normal(9000),
+           synth.seismic_traces(1, 12000, seed=7)[0]]
+    launcher = EMDKernelConfig(max_extrema=4096).build()
+    offsets = np.zeros(len(sig) + 1, np.int64)
+    offsets[1:] = np.cumsum([len(s) for s in sig])
+    out = launcher.run_ragged(torch.from_numpy(np.concatenate(sig)).cuda(),
+                              torch.from_numpy(offsets), threads=threads, tile_slots=786,
+                              persistent=True)
+    torch.cuda.synchronize()
+    flat = out[0].cpu().numpy()
+    gpu = dict(imfs=[flat[offsets[b] * 16: offsets[b] * 16 + 16 * len(s)].reshape(16, len(s))
+                     for b, s in enumerate(sig)],
+               n_imfs=out[2].cpu().numpy(), overflow=out[3].cpu().numpy(),
+               debug_n=out[4].cpu().numpy(), debug_extno=out[5].cpu().numpy(),
+               debug_nzm=out[6].cpu().numpy())
+    _assert_parity(sig, gpu, _oracle(sig, max_extrema=4096))
