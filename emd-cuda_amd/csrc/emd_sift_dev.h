@@ -147,7 +147,8 @@ __device__ __forceinline__ int round_half_even_mid(int s) {
 // levels inside 16-lane rows: quad permutations for lane^1 and lane^2, then row mirrors
 // (lane i <- 7-i, lane i <- 15-i), which pair each lane with the other half of its group;
 // once a group holds one value in every lane that is the same pairing as lane^4 / lane^8.
-// The last two levels combine the four rows' values read into scalars.
+// The last two levels either read the four rows' values into scalars (max/min) or use
+// row broadcasts (sums).
 constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
 constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
 constexpr int kDppHalfMirror = 0x141; // row_half_mirror
@@ -190,18 +191,10 @@ __device__ __forceinline__ void tile_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Perfect binary tree over the 64 lanes (lane order); every lane gets the same bits:
-// level d adds the partner at lane^d, i.e. ((x0+x1)+(x2+x3))+... (fp addition commutes,
-// so both partners form bit-identical sums).
-__device__ __forceinline__ double wave_tree_sum(double v) {
-  v += dpp_d<kDppXor1>(v);
-  v += dpp_d<kDppXor2>(v);
-  v += dpp_d<kDppHalfMirror>(v);
-  v += dpp_d<kDppMirror>(v);
-  return (lane_d(v, 0) + lane_d(v, 16)) + (lane_d(v, 32) + lane_d(v, 48));
-}
-// The same tree with the last two levels as row broadcasts; only lane 63 holds the sum
-// (row 3 adds row 2's value, then rows 0+1 from lane 31: ((r0+r1)+(r2+r3)) bit for bit).
+// Perfect binary tree over the 64 lanes (lane order): level d adds the partner at lane^d,
+// i.e. ((x0+x1)+(x2+x3))+... (fp addition commutes, so both partners form bit-identical
+// sums).  The first four levels run on every lane; the last two are row broadcasts, so
+// only lane 63 holds the sum (row 3 adds row 2's value, then rows 0+1 from lane 31).
 __device__ __forceinline__ double wave_tree_sum_last(double v) {
   v += dpp_d<kDppXor1>(v);
   v += dpp_d<kDppXor2>(v);
@@ -492,7 +485,11 @@ __device__ __forceinline__ KnotEnds load_ends(const Tile& t, int tb0, int tb1, i
   c.xi = 0;
   c.yv = 0.0;
   if (lane < 4 * F) {
-    const int side = lane / (2 * F), half = (lane / F) & 1, k = lane % F;
+    // lane = side * 2F + half * F + k, by compares (F is a runtime value)
+    const int side = lane >= 2 * F ? 1 : 0;
+    const int r = lane - side * 2 * F;
+    const int half = r >= F ? 1 : 0;
+    const int k = r - half * F;
     const int n = side ? nn : nm;
     const int slot = max(0, min(half ? n - F + k : k, n - 1));
     const int tb = side ? tb1 : tb0;
@@ -759,7 +756,10 @@ __device__ __forceinline__ void sweep_row(const Tile& t, int k0, int m, int j, b
 __device__ __forceinline__ void spline_system_wave(const Tile& t, int side, int k0, int m,
                                                    int lane) {
   const int K = min(64, m / 2);  // chunks of >= 2 rows
-  const int base = m / K, rem = m % K;
+  // m / K and m % K without a division: K = 64 for m >= 128; otherwise K = m / 2, which
+  // leaves chunks of 2 (plus one of 3 when m is odd), except m = 3 (one chunk of 3)
+  const int base = m >= 128 ? (m >> 6) : (m == 3 ? 3 : 2);
+  const int rem = m >= 128 ? (m & 63) : (m == 3 ? 0 : (m & 1));
   const bool in_regs = base + (rem > 0 ? 1 : 0) - 2 <= kRegRows;  // interior rows per lane
   const bool act = lane < K;
   const int s = act ? lane * base + min(lane, rem) : 0;
@@ -1021,7 +1021,7 @@ __device__ __forceinline__ double blocks_tree(const double* v, int nblk) {
         ((b + 2 < nblk ? x2 : 0.0) + (b + 3 < nblk ? x3 : 0.0));
   }
   // lanes past the leaves hold zeros: the 64-leaf tree equals the tree over P leaves
-  return wave_tree_sum(a);
+  return lane_d(wave_tree_sum_last(a), 63);
 }
 
 struct SiftStats {
