@@ -241,6 +241,16 @@ def bench_costmodel(args):
     dv, doff = torch.from_numpy(values).cuda(), torch.from_numpy(offsets).cuda()
     tuner = Autotuner(cfg, cache_path=args.tune_cache)
     res = {}
+    # the plain kernel over the same CSR batch: one workgroup per trace, batch order
+    launcher.run_ragged(dv, offsets, persistent=False)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    pout = launcher.run_ragged(dv, offsets, persistent=False)
+    torch.cuda.synchronize()
+    res["plain"] = (time.perf_counter() - ts, [pout[k].cpu().numpy() for k in range(8)[2:]])
+    _progress(f"held-out plain: {res['plain'][0]:.2f}s")
+    del pout
+    torch.cuda.empty_cache()
     for name, model in (("fifo", None), ("learned", cm)):
         p = PackedEMD(cfg, model, tuner).prepare(dv, offsets)
         p.run()
@@ -256,7 +266,8 @@ def bench_costmodel(args):
     out = res["learned"][1]  # n_imfs, overflow, debug_n, debug_extno, debug_nzm, stats
     true = costmodel.targets_from(out[5], out[0])
     pred = cm.predict(costmodel.features(dv, doff))
-    same = all(np.array_equal(a, b) for a, b in zip(res["fifo"][1][:5], out[:5]))
+    same = all(np.array_equal(a, b) for a, b in zip(res["fifo"][1][:5], out[:5])) and \
+        all(np.array_equal(a, b) for a, b in zip(res["plain"][1][:5], out[:5]))
     B = len(traces)
     line = {
         "metric": "cost-model MAPE on held-out traces (sifts, n_imfs, peak extrema)",
@@ -270,10 +281,12 @@ def bench_costmodel(args):
         "mape_white": costmodel.mape(pred[kinds == 1], true[kinds == 1]),
         "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1], "epochs": args.epochs,
         "train_decompose_and_features_s": gen_train_s, "fit_s": fit_s,
+        "eval_plain_traces_per_s": B / res["plain"][0],
         "eval_fifo_traces_per_s": B / res["fifo"][0],
         "eval_learned_traces_per_s": B / res["learned"][0],
         "learned_speedup_vs_fifo": res["fifo"][0] / res["learned"][0],
-        "identical_fifo_learned": bool(same),
+        "learned_speedup_vs_plain": res["plain"][0] / res["learned"][0],
+        "identical_plain_fifo_learned": bool(same),
         "overflow_traces": int((out[1] != 0).sum()),
     }
     print(json.dumps(line), flush=True)
