@@ -1059,9 +1059,11 @@ __device__ __forceinline__ void envelope_pair(const Tile& t, const SiftShared<WG
     const double hh = double(xb - xa);
     const double A = double(xb - ii) * ih;
     const double B = double(ii - xa) * ih;
-    const double e = A * t.Y[jj] + B * t.Y[jj + 1] +
-                     ((A * A * A - A) * t.M[jj] + (B * B * B - B) * t.M[jj + 1]) *
-                         (hh * hh * kSixth);
+    // A y0 + B y1 + h^2/6 ((A^3 - A) M0 + (B^3 - B) M1), with fused multiply-adds
+    const double ca = __builtin_fma(A * A, A, -A), cb = __builtin_fma(B * B, B, -B);
+    const double curv = __builtin_fma(ca, t.M[jj], cb * t.M[jj + 1]);
+    const double lin = __builtin_fma(A, t.Y[jj], B * t.Y[jj + 1]);
+    const double e = __builtin_fma(curv, hh * hh * kSixth, lin);
     if (q == 0) e0 = e; else e1 = e;
   }
 }
