@@ -874,10 +874,13 @@ __device__ __forceinline__ void spline_system_wave(const Tile& t, int side, int 
       ddn = __shfl_down(de, dd);
     }
     const bool hu = lane >= dd, hd = lane + dd < 64;
-    const double nA = hu ? -(al * au) : 0.0;
-    const double nC = hd ? -(ga * gd) : 0.0;
-    const double nB = 1.0 - (hu ? al * gu : 0.0) - (hd ? ga * ad : 0.0);
-    const double nD = de - (hu ? al * du : 0.0) - (hd ? ga * ddn : 0.0);
+    // neighbours past the ends contribute nothing: their coefficients are taken as 0
+    const double au0 = hu ? au : 0.0, gu0 = hu ? gu : 0.0, du0 = hu ? du : 0.0;
+    const double ad0 = hd ? ad : 0.0, gd0 = hd ? gd : 0.0, dd0 = hd ? ddn : 0.0;
+    const double nA = -(al * au0);
+    const double nC = -(ga * gd0);
+    const double nB = __builtin_fma(-ga, ad0, __builtin_fma(-al, gu0, 1.0));
+    const double nD = __builtin_fma(-ga, dd0, __builtin_fma(-al, du0, de));
     ib = 1.0 / nB;
     al = nA * ib;
     ga = nC * ib;
@@ -1095,7 +1098,7 @@ __device__ __forceinline__ void envelope_block(const Work& wk, const Tile& t,
     }
     // i + 1 < 64 * nbcap always (i is even and < N), so the pair read stays in the work area
     const double old0 = wk.imf[i], old1 = wk.imf[i + 1];
-    const double nw0 = old0 - 0.5 * (emax0 + emin0);
+    const double nw0 = __builtin_fma(-0.5, emax0 + emin0, old0);
     const double d0 = nw0 - old0;
     const double r0 = d0 * recip(nw0);
     ln = nw0 * nw0;
@@ -1105,12 +1108,12 @@ __device__ __forceinline__ void envelope_block(const Work& wk, const Tile& t,
     nmin = fmin(nmin, nw0);
     wk.imf[i] = nw0;
     if (i + 1 < N) {
-      const double nw1 = old1 - 0.5 * (emax1 + emin1);
+      const double nw1 = __builtin_fma(-0.5, emax1 + emin1, old1);
       const double d1 = nw1 - old1;
       const double r1 = d1 * recip(nw1);
-      ln = ln + nw1 * nw1;
-      ld = ld + d1 * d1;
-      ls = ls + r1 * r1;
+      ln = __builtin_fma(nw1, nw1, ln);
+      ld = __builtin_fma(d1, d1, ld);
+      ls = __builtin_fma(r1, r1, ls);
       nmax = fmax(nmax, nw1);
       nmin = fmin(nmin, nw1);
       wk.imf[i + 1] = nw1;
