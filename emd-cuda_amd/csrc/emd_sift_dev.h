@@ -88,7 +88,8 @@ struct SiftShared {
 // bookkeeping, laid out for the launch's longest trace.
 struct Work {
   double* imf;      // [nblk*64] current IMF
-  double* bsum;     // [4][nblk] block partial sums
+  double* bsum;     // [4][nblk] block partial sums: the sift's three sums (rows 0-2), the
+                    // accept pass's |residual| (row 0) and an IMF start's sum of squares (row 3)
   uint64_t* mmax;   // [nblk] maxima masks of the last detection
   uint64_t* mmin;   // [nblk] minima masks
   int* pmax;        // [nblk] exclusive prefix counts of maxima over blocks
