@@ -403,32 +403,41 @@ def main():
     launcher = cfg.build()
     results = {}
 
-    launcher.set_timing(True)
+    events = []
+
+    def sift_call(call):
+        # HIP events on the launch stream (torch's current stream, which the launcher uses)
+        # around the sift launch; read after the timed loop, so no step waits on them
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        out = call()
+        e1.record()
+        events.append((e0, e1))
+        return out
 
     def timed(fn):
         for _ in range(args.warmup):
             fn()
         torch.cuda.synchronize()
-        kernel_ms = []
+        events.clear()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             out = fn()
-            # HIP events the library records on this stream around the sift launch
-            kernel_ms.append(launcher.kernel_time_ms())
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        kernel_ms = [a.elapsed_time(b) for a, b in events]
         return out, wall / args.steps * 1e3, float(np.mean(kernel_ms))
 
     if args.mode in ("both", "plain"):
-        out, ms_wall, ms_ev = timed(lambda: launcher.run(dx))
+        out, ms_wall, ms_ev = timed(lambda: sift_call(lambda: launcher.run(dx)))
         n_imfs = out[1].cpu().numpy()
         stats = launcher.last_stats.cpu().numpy()
         results["plain"] = dict(ms=ms_wall, kernel_ms=ms_ev, n_imfs=n_imfs, stats=stats,
                                 out=out)
     if args.mode in ("both", "fifo"):
         def fifo():
-            return launcher.run_ragged(dx.view(-1), offsets, threads=args.threads,
-                                       persistent=True)
+            return sift_call(lambda: launcher.run_ragged(dx.view(-1), offsets,
+                                                         threads=args.threads, persistent=True))
         out, ms_wall, ms_ev = timed(fifo)
         results["fifo"] = dict(ms=ms_wall, kernel_ms=ms_ev, n_imfs=out[2].cpu().numpy(),
                                stats=out[7].cpu().numpy(), out=out)
@@ -447,8 +456,9 @@ def main():
             plan = cm.plan(f, dlen, nbsym=cfg.nbsym)
             tile = args.tile or plan.tile_slots
             plan_tile.append(tile)
-            return launcher.run_ragged(dx.view(-1), offsets, order=plan.order,
-                                       threads=args.threads, tile_slots=tile, persistent=True)
+            return sift_call(lambda: launcher.run_ragged(dx.view(-1), offsets, order=plan.order,
+                                                         threads=args.threads, tile_slots=tile,
+                                                         persistent=True))
         out, ms_wall, ms_ev = timed(packed)
         results["packed"] = dict(ms=ms_wall, kernel_ms=ms_ev, n_imfs=out[2].cpu().numpy(),
                                  stats=out[7].cpu().numpy(), out=out)

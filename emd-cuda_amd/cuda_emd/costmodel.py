@@ -132,6 +132,7 @@ class CostModel:
         self.tsd = np.ones(3)
         self.history: list[float] = []
         self.sift_rank = float("nan")  # held-out rank correlation of predicted sifts
+        self._tstd = None  # (tsd, tmu) on the device, made by the first plan after a fit/load
 
     @property
     def orders_by_sifts(self) -> bool:
@@ -187,6 +188,7 @@ class CostModel:
         self.norm = torch.cat([mu, sd]).contiguous()
         t = _transform(np.asarray(raw_targets, dtype=np.float64))
         self.tmu, self.tsd = t.mean(0), np.maximum(t.std(0), 1e-3)
+        self._tstd = None
         ts = torch.tensor((t - self.tmu) / self.tsd, dtype=torch.float32, device=self.device)
         tw = torch.tensor(target_weight, dtype=torch.float32, device=self.device)
         g = torch.Generator(device=self.device)
@@ -220,8 +222,10 @@ class CostModel:
         needs."""
         torch = _torch()
         y = self.forward_std(feats).double()
-        tsd = torch.tensor(self.tsd, dtype=torch.float64, device=self.device)
-        tmu = torch.tensor(self.tmu, dtype=torch.float64, device=self.device)
+        if self._tstd is None:
+            self._tstd = (torch.tensor(self.tsd, dtype=torch.float64, device=self.device),
+                          torch.tensor(self.tmu, dtype=torch.float64, device=self.device))
+        tsd, tmu = self._tstd
         y = y * tsd + tmu
         L = lengths if isinstance(lengths, torch.Tensor) else torch.from_numpy(
             np.asarray(lengths, dtype=np.float64))
@@ -254,5 +258,6 @@ class CostModel:
         cm.params = torch.tensor(d["params"], dtype=torch.float32, device=cm.device)
         cm.norm = torch.tensor(d["norm"], dtype=torch.float32, device=cm.device)
         cm.tmu, cm.tsd = np.array(d["tmu"]), np.array(d["tsd"])
+        cm._tstd = None
         cm.sift_rank = float(d["sift_rank"]) if "sift_rank" in d.files else float("nan")
         return cm
