@@ -1,13 +1,14 @@
 // emd_sift_dev.h -- batched Empirical Mode Decomposition (PyEMD EMD() defaults) for gfx950.
 //
 // One workgroup decomposes one trace at a time.  The current IMF lives in LDS (8 bytes per
-// sample) and is processed in blocks of 64 samples, one sample per lane: extrema are found
-// with wave64 ballots and compacted by popcount over a block-prefix scan, the same masks
-// give every sample its envelope interval, and every reduction is a 64-lane butterfly per
-// block followed by a perfect binary tree over the block sums.  The knot tile (positions,
-// values and the four tridiagonal arrays of the joint max+min envelope solve) follows the
-// IMF in LDS; a sift whose knots outgrow the launch's LDS tile uses the workgroup's own
-// tile region in global memory instead (same code, same arithmetic).
+// sample) and is processed in blocks of 64 samples, one sample per lane (two adjacent
+// samples per lane in 128-sample blocks for the envelope pass): extrema are found with
+// wave64 ballots and compacted by popcount over a block-prefix scan, the same masks give
+// every sample its envelope interval, and every reduction is a 64-lane butterfly per block
+// followed by a perfect binary tree over the block sums.  The knot tile (position, value,
+// second derivative and 1/h per knot slot, both envelopes) follows the IMF in LDS; a sift
+// whose knots outgrow the launch's LDS tile uses the workgroup's own tile region in global
+// memory instead (same code, same arithmetic).
 //
 // Reference counterparts (SeismicFlow/emd-cuda kernels.py):
 //   detect()          <- find_extrema_serial / find_extrema_parallel   kernels.py:115-349
@@ -21,8 +22,9 @@
 // next_pow2(N) samples in index order (butterfly inside each aligned 64-sample block, then
 // a perfect tree over the blocks; zero leaves past N do not change it), the tridiagonal
 // solve's operation order depends only on row indices, and the build uses
-// -ffp-contract=off.  Results are therefore bit-identical for every workgroup size, LDS
-// tile, tile placement, queue order and kernel (plain or persistent).
+// -ffp-contract=off with the few fused multiply-adds written out.  Results are therefore
+// bit-identical for every workgroup size, LDS tile, tile placement, queue order and kernel
+// (plain or persistent).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -47,8 +49,9 @@ enum SplineKind { kConst = 0, kLinear = 1, kThreePoint = 2, kNotAKnot = 3 };
 
 struct Tile {
   double* Y;   // knot values
-  double* M;   // second derivatives by knot slot (after the solve)
-  double* H;   // 1/h of the interval that starts at each knot slot (after the solve)
+  double* M;   // chord slope of the interval that starts at each knot slot, overwritten
+               // by the solve with the second derivative at the slot
+  double* H;   // 1/h of the interval that starts at each knot slot
   int* X;      // knot positions (sample index; mirrored knots may be <0 or >N-1)
   double* R;   // solver rows of systems longer than 64 * (kRegRows + 2) rows, in the
                // workgroup's global region: side s, array a (g, u, v, w) at
