@@ -86,13 +86,28 @@ __global__ __launch_bounds__(256) void features_kernel(const double* __restrict_
     sn[t] = s;
   }
   double ext = 0, zc = 0, ss = 0;
-  for (int i = t; i < n; i += 256) {
-    const double v = x[i];
-    ss += v * v;
-    if (i + 1 < n) {
-      const double w = x[i + 1];
-      if (v * w < 0) zc += 1;
-      if (i >= 1 && (v - x[i - 1]) * (w - v) < 0) ext += 1;
+  // four strided samples per step, all their loads issued before the first use
+  for (int i0 = t; i0 < n; i0 += 4 * 256) {
+    double vv[4], ww[4], pp[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + 256 * u;
+      vv[u] = i < n ? x[i] : 0.0;
+      ww[u] = i + 1 < n ? x[i + 1] : 0.0;
+      pp[u] = (i >= 1 && i < n) ? x[i - 1] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + 256 * u;
+      if (i < n) {
+        const double v = vv[u];
+        ss += v * v;
+        if (i + 1 < n) {
+          const double w = ww[u];
+          if (v * w < 0) zc += 1;
+          if (i >= 1 && (v - pp[u]) * (w - v) < 0) ext += 1;
+        }
+      }
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
