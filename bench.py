@@ -48,11 +48,11 @@ def algorithmic_bytes(n_imfs: np.ndarray, lengths: np.ndarray) -> float:
     return float(np.sum(8.0 * L + 8.0 * L * n_imfs + 4 * (2 + 3 * 16 + 4)))
 
 
-def measured_traffic():
-    """HBM bytes per launch of the persistent kernel on this workload, from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (counters cannot be read inside the timed run)."""
+def measured_traffic(name="emd_sift_kernels_seismic_16384x2001.md"):
+    """HBM bytes per launch of a kernel on its workload, from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (counters cannot be read inside the timed run)."""
     import re
-    path = os.path.join(ROOT, "profiles", "emd_sift_kernels_seismic_16384x2001.md")
+    path = os.path.join(ROOT, "profiles", name)
     try:
         m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", open(path).read())
     except OSError:
@@ -331,6 +331,70 @@ def bench_stress(args):
     print(json.dumps(line), flush=True)
 
 
+def segy_image(B: int, N: int, seed: int) -> np.ndarray:
+    """A SEG-Y file image (rev 1, IBM float samples, 4 ms) of B traces x N samples whose
+    samples are random normalised IBM words of magnitude 16^-3 .. 16^2."""
+    rng = np.random.default_rng(seed)
+    img = np.zeros(3600 + B * (240 + 4 * N), dtype=np.uint8)
+    img[3216:3218] = (4000 >> 8, 4000 & 255)          # sample interval (us)
+    img[3220:3222] = (N >> 8, N & 255)                # samples per trace
+    img[3225] = 1                                     # IBM float
+    tr = img[3600:].reshape(B, 240 + 4 * N)
+    tr[:, 114], tr[:, 115] = N >> 8, N & 255
+    words = ((rng.integers(0, 2, (B, N), dtype=np.uint32) << 31)
+             | (rng.integers(61, 67, (B, N), dtype=np.uint32) << 24)
+             | rng.integers(0x100000, 0x1000000, (B, N), dtype=np.uint32))
+    tr[:, 240:] = words.astype(">u4").view(np.uint8).reshape(B, 4 * N)
+    return img
+
+
+def bench_segy(args):
+    """SEG-Y in: the decode kernel on a file image already in HBM (16384 traces x N=2001,
+    IBM float), and the same with the pinned host -> device copy of the trace bytes."""
+    import torch
+    from cuda_emd import traceio
+    B, N = args.batch, args.n
+    img = segy_image(B, N, seed=6)
+    lay = traceio.scan_segy(img)
+    host = torch.from_numpy(img[lay.data_start:]).pin_memory()
+    dev = host.to("cuda")
+    out = torch.empty((B, N), dtype=torch.float64, device="cuda")
+    for _ in range(args.warmup):
+        traceio.decode_segy(dev, lay, out=out)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(args.steps):
+        traceio.decode_segy(dev, lay, out=out)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dev.copy_(host, non_blocking=True)
+        traceio.decode_segy(dev, lay, out=out)
+    torch.cuda.synchronize()
+    ms_h2d = (time.perf_counter() - t0) * 1e3 / args.steps
+    alg = 12.0 * N * B                      # 4 B read + 8 B written per sample
+    achieved = alg / (ms * 1e-3) / 1e9
+    line = {
+        "metric": "SEG-Y traces decoded to fp64 per second (IBM float, N=2001), 1 GPU",
+        "value": B / (ms * 1e-3), "unit": "traces/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32->f64", "data": "synthetic",
+        "config": {"workload": f"SEG-Y image, {B} traces x N={N}, IBM float, seed 6",
+                   "traces": B, "N": N, "file_bytes": int(img.size)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("segy_decode_16384x2001.md"),
+                     "algorithmic_bytes": alg},
+        "h2d_inclusive_traces_per_s": B / (ms_h2d * 1e-3),
+        "h2d_inclusive_ms": ms_h2d,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -352,14 +416,15 @@ def main():
     ap.add_argument("--train", type=int, default=8192,
                     help="seismic traces (seed 3) the cost model is trained on before timing")
     ap.add_argument("--epochs", type=int, default=30)
-    ap.add_argument("--workload", choices=["seismic", "ragged", "costmodel", "stress"],
+    ap.add_argument("--workload", choices=["seismic", "ragged", "costmodel", "stress", "segy"],
                     default="seismic",
                     help="seismic: the headline 16384 x N=2001 traces; ragged: the ragged mix "
                          "(N log-uniform in [256, 8192], seed 2; --batch sets B, default 65536); "
                          "costmodel: train on 200k seismic decompositions (seed 3), evaluate "
                          "on 262144 held-out traces (seed 4, 5%% white noise at N=8192); "
                          "stress: 262144 x N=4096 (seed 5) streamed from host in captured "
-                         "chunks, max_extrema 2048, compacted output")
+                         "chunks, max_extrema 2048, compacted output; segy: decode of a "
+                         "16384 x N=2001 IBM-float SEG-Y image in HBM")
     ap.add_argument("--train-total", type=int, default=200000)
     ap.add_argument("--eval-total", type=int, default=262144)
     ap.add_argument("--tune-cache", default=None, help="autotuner JSON cache (ragged)")
@@ -385,6 +450,8 @@ def main():
         return bench_costmodel(args)
     if args.workload == "stress":
         return bench_stress(args)
+    if args.workload == "segy":
+        return bench_segy(args)
 
     from cuda_emd import synth
 

@@ -76,6 +76,9 @@ PROTOTYPES = {
     "emd_mlp_backward": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
     "emd_mlp_adam": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_float,
                                     ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp]),
+    "emd_segy_sample_bytes": (ctypes.c_int, [ctypes.c_int]),
+    "emd_segy_decode": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int, _vp, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, _vp, _vp]),
 }
 
 _lib = None

@@ -196,6 +196,21 @@ int emd_mlp_backward(const float* params, const float* norm, const float* feats,
 int emd_mlp_adam(float* params, float* m, float* v, const float* grad, int step, float lr,
                  float beta1, float beta2, float eps, void* stream);
 
+/* ---- SEG-Y trace samples ----------------------------------------------------------------
+ * Decodes big-endian SEG-Y samples in device memory to fp64 rows for emd_run_padded /
+ * emd_run_ragged.  Formats (binary header bytes 3225-3226): 1 IBM float, 2 int32, 3 int16,
+ * 5 IEEE float, 6 IEEE double, 8 int8; every conversion is exact.  Trace b's header starts
+ * at bytes + trace_pos[b] (or b * trace_stride when trace_pos is NULL) and its samples
+ * header_bytes later.  Output: out[b*N + i] (offsets NULL), or out[offsets[b] + i] for
+ * offsets[b+1] - offsets[b] samples (CSR; needs trace_pos).  Unaligned traces are read
+ * byte by byte.  emd_segy_sample_bytes gives a format's sample size (0 = unsupported).
+ * No reference counterpart: the reference takes fp64 device arrays (emd.py:60-69) of
+ * seismic traces sampled at 4 ms (emd.py:28-30). */
+int emd_segy_sample_bytes(int format);
+int emd_segy_decode(const void* bytes, const int64_t* trace_pos, int64_t trace_stride,
+                    int header_bytes, const int64_t* offsets, int N, int B, int format,
+                    double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
