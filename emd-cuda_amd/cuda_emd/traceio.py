@@ -5,6 +5,7 @@
   there (``emd_segy_decode``) to the float64 rows ``GPUBatchEMD`` / ``run_ragged`` take:
   a [B, N] batch when every trace has the file's sample count, else a CSR batch
   (values + int64 offsets).
+  ``iter_segy`` does the same chunk by chunk for files larger than memory.
 - CSR batches and compacted IMF rows on disk: ``save_csr`` / ``load_csr`` and
   ``save_compacted`` / ``load_compacted`` (``.npz``, no pickled objects).
 
@@ -181,6 +182,53 @@ def read_segy(path: str, device=None) -> SegyTraces:
     else:
         out.values, out.offsets = res
     return out
+
+
+def iter_segy(path: str, chunk: int = 16384, device=None):
+    """Read a SEG-Y file chunk by chunk (files larger than host or device memory): yields
+    (first trace index, SegyTraces) for each run of up to ``chunk`` traces, in file order.
+    The file is memory-mapped; each chunk's bytes are staged through one of two pinned
+    buffers and copied and decoded on the current stream, so filling the next chunk on the
+    host overlaps the device's copy and decode of this one.  A staging buffer is refilled
+    only after the copy out of it has completed."""
+    torch = _torch()
+    if chunk < 1:
+        raise ValueError(f"chunk must be >= 1, got {chunk}")
+    mm = np.memmap(path, dtype=np.uint8, mode="r")
+    layout = scan_segy(mm)
+    dev = torch.device(device) if device is not None else torch.device(
+        "cuda", torch.cuda.current_device())
+    st = torch.cuda.current_stream(dev)
+    body = mm[layout.data_start:]
+    end = np.append(layout.trace_pos[1:], body.size) if layout.n_traces else np.zeros(0)
+    slots = [None, None]
+    copied = [None, None]
+    for k, t0 in enumerate(range(0, layout.n_traces, chunk)):
+        t1 = min(t0 + chunk, layout.n_traces)
+        a, b = int(layout.trace_pos[t0]), int(end[t1 - 1])
+        slot = k & 1
+        if slots[slot] is None or slots[slot].numel() < b - a:
+            if copied[slot] is not None:
+                copied[slot].synchronize()
+            slots[slot] = torch.empty(b - a, dtype=torch.uint8).pin_memory()
+        elif copied[slot] is not None:
+            copied[slot].synchronize()
+        stage = slots[slot][:b - a]
+        stage.numpy()[:] = body[a:b]
+        dev_bytes = stage.to(dev, non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(st)
+        copied[slot] = ev
+        sub = SegyLayout(layout.format, layout.sample_bytes, layout.dt_us, layout.n_samples,
+                         layout.data_start + a, layout.trace_pos[t0:t1] - a,
+                         layout.lengths[t0:t1], layout.fixed)
+        res = decode_segy(dev_bytes, sub)
+        out = SegyTraces(layout=sub, dt=layout.dt_us * 1e-6)
+        if sub.fixed:
+            out.signals = res
+        else:
+            out.values, out.offsets = res
+        yield t0, out
 
 
 def _host(a) -> np.ndarray:

@@ -104,3 +104,28 @@ def test_segy_batch_decomposes_like_float_input(tmp_path):
     db = emd.last_debug
     for u, v in zip(list(a) + list(da), list(b) + list(db)):
         assert torch.equal(u, v)
+
+
+@pytest.mark.parametrize("fmt,ragged", [(1, False), (3, True), (5, False)])
+def test_iter_segy_chunks_match_whole_read(tmp_path, fmt, ragged):
+    rng = np.random.default_rng(30 + fmt)
+    lengths = [int(n) for n in rng.integers(1, 900, 37)] if ragged else [517] * 37
+    print(f"iter_segy fmt={fmt} ragged={ragged}", flush=True)
+    data, _ = segy_files.random_segy(fmt, lengths, seed=40 + fmt, ns=lengths[0])
+    p = tmp_path / "it.sgy"
+    p.write_bytes(data)
+    whole = traceio.read_segy(str(p))
+    seen = 0
+    for t0, part in traceio.iter_segy(str(p), chunk=10):
+        assert t0 == seen
+        n = part.layout.n_traces
+        if ragged:
+            o = whole.offsets.cpu().numpy()
+            got = part.values.cpu().numpy()
+            assert np.array_equal(part.offsets.cpu().numpy(), o[t0:t0 + n + 1] - o[t0])
+            assert np.array_equal(_bits(got), _bits(whole.values.cpu().numpy()[o[t0]:o[t0 + n]]))
+        else:
+            assert np.array_equal(_bits(part.signals.cpu().numpy()),
+                                  _bits(whole.signals.cpu().numpy()[t0:t0 + n]))
+        seen += n
+    assert seen == len(lengths)
