@@ -48,16 +48,21 @@ def algorithmic_bytes(n_imfs: np.ndarray, lengths: np.ndarray) -> float:
     return float(np.sum(8.0 * L + 8.0 * L * n_imfs + 4 * (2 + 3 * 16 + 4)))
 
 
-def measured_traffic(name="emd_sift_kernels_seismic_16384x2001.md"):
-    """HBM bytes per launch of a kernel on its workload, from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (counters cannot be read inside the timed run)."""
+def profile_values(name="emd_sift_kernels_seismic_16384x2001.md") -> dict:
+    """The ``key: number`` lines of a committed rocprofv3 summary under profiles/
+    (counters cannot be read inside the timed run)."""
     import re
-    path = os.path.join(ROOT, "profiles", name)
     try:
-        m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", open(path).read())
+        text = open(os.path.join(ROOT, "profiles", name)).read()
     except OSError:
-        return None
-    return float(m.group(1)) if m else None
+        return {}
+    return {k: float(v) for k, v in re.findall(r"^([a-z0-9_]+):\s*([0-9.]+)\s*$", text, re.M)}
+
+
+def measured_traffic(name="emd_sift_kernels_seismic_16384x2001.md"):
+    """HBM bytes per launch of a kernel on its workload, from its FETCH_SIZE/WRITE_SIZE
+    summary."""
+    return profile_values(name).get("traffic_bytes_per_launch")
 
 
 def _cpu_one(x):
@@ -578,6 +583,25 @@ def main():
                      "flops_per_launch": flops,
                      "algorithmic_hbm_bytes": algorithmic_bytes(r["n_imfs"], lengths)},
     }
+    if head == "packed" and B == 16384:
+        # Hardware counters of the same launch (committed rocprofv3 passes): counted fp64
+        # operations over this run's kernel time, LDS activity, and resident-slot occupancy.
+        pv = profile_values()
+        if "counted_fp64_ops_per_launch" in pv:
+            kernel_ms = r["kernel_ms"]
+            counted = pv["counted_fp64_ops_per_launch"] / (kernel_ms * 1e-3) / 1e12
+            line["counters"] = {
+                "counted_fp64_tflops": counted,
+                "counted_fp64_frac": counted / FP64_PEAK_TFLOPS,
+                "fp64_issue_per_cu_cycle": pv.get("fp64_issue_per_cu_cycle"),
+                "lds_active_frac": pv.get("lds_active_frac"),
+                "lds_bank_conflict_frac": pv.get("lds_bank_conflict_frac"),
+                "lds_tb_per_s": pv.get("lds_bytes_per_launch", 0.0) / (kernel_ms * 1e-3) / 1e12,
+                "mean_waves_per_cu": pv.get("mean_waves_per_cu"),
+                "resident_waves_per_cu": pv.get("resident_waves_per_cu"),
+                "queue_idle_frac": pv.get("queue_idle_frac"),
+                "source": "profiles/emd_sift_kernels_seismic_16384x2001.md",
+            }
     if "plain" in results:
         line["plain_traces_per_s"] = B / (results["plain"]["ms"] * 1e-3)
         line["plain_ms_per_step"] = results["plain"]["ms"]
