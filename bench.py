@@ -597,6 +597,9 @@ def main():
                 "lds_active_frac": pv.get("lds_active_frac"),
                 "lds_bank_conflict_frac": pv.get("lds_bank_conflict_frac"),
                 "lds_tb_per_s": pv.get("lds_bytes_per_launch", 0.0) / (kernel_ms * 1e-3) / 1e12,
+                "hbm_tb_per_s": pv.get("traffic_bytes_per_launch", 0.0) / (kernel_ms * 1e-3) / 1e12,
+                "hbm_frac": pv.get("traffic_bytes_per_launch", 0.0) / (kernel_ms * 1e-3) / 1e9
+                / HBM_PEAK_GBS,
                 "mean_waves_per_cu": pv.get("mean_waves_per_cu"),
                 "resident_waves_per_cu": pv.get("resident_waves_per_cu"),
                 "queue_idle_frac": pv.get("queue_idle_frac"),
