@@ -18,6 +18,7 @@ trace at 3221-3222, sample format code at 3225-3226, extended textual header cou
 from __future__ import annotations
 
 import ctypes
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -168,11 +169,21 @@ def read_segy(path: str, device=None) -> SegyTraces:
     """Read a SEG-Y file: headers on the host, one pinned copy of the trace bytes to the
     device, every sample decoded there."""
     torch = _torch()
-    buf = np.fromfile(path, dtype=np.uint8)
-    layout = scan_segy(buf)
+    layout = scan_segy(np.memmap(path, dtype=np.uint8, mode="r"))
     dev = torch.device(device) if device is not None else torch.device(
         "cuda", torch.cuda.current_device())
-    host = torch.from_numpy(buf[layout.data_start:]).pin_memory()
+    # The trace bytes go straight from the file into pinned memory (one host copy).
+    with open(path, "rb") as f:
+        f.seek(layout.data_start)
+        host = torch.empty(os.fstat(f.fileno()).st_size - layout.data_start,
+                           dtype=torch.uint8, pin_memory=True)
+        view = memoryview(host.numpy())
+        got = 0
+        while got < len(view):
+            n = f.readinto(view[got:])
+            if not n:
+                raise ValueError(f"{path} shrank while being read")
+            got += n
     dev_bytes = host.to(dev, non_blocking=True)
     res = decode_segy(dev_bytes, layout)
     torch.cuda.current_stream(dev).synchronize()
@@ -210,7 +221,7 @@ def iter_segy(path: str, chunk: int = 16384, device=None):
         if slots[slot] is None or slots[slot].numel() < b - a:
             if copied[slot] is not None:
                 copied[slot].synchronize()
-            slots[slot] = torch.empty(b - a, dtype=torch.uint8).pin_memory()
+            slots[slot] = torch.empty(b - a, dtype=torch.uint8, pin_memory=True)
         elif copied[slot] is not None:
             copied[slot].synchronize()
         stage = slots[slot][:b - a]
