@@ -433,6 +433,9 @@ def main():
     ap.add_argument("--tile", type=int, default=0,
                     help="LDS knot slots of the packed path (0: sized by the cost model)")
     ap.add_argument("--threads", type=int, default=256)
+    ap.add_argument("--serial-plan", action="store_true",
+                    help="packed path: compute each step's plan on the launch stream, after "
+                         "the previous step's sift (default: on a planning stream)")
     ap.add_argument("--train", type=int, default=8192,
                     help="seismic traces (seed 3) the cost model is trained on before timing")
     ap.add_argument("--epochs", type=int, default=30)
@@ -537,10 +540,24 @@ def main():
 
         dlen = torch.from_numpy(lengths.astype(np.float64)).cuda()
 
+        plan_stream = None if args.serial_plan else torch.cuda.Stream()
+
         def packed():
-            # the whole learned path: features -> MLP -> queue order + tile -> sift launch
-            f = costmodel.features(dx.view(-1), doff)
-            plan = cm.plan(f, dlen, nbsym=cfg.nbsym)
+            # the whole learned path: features -> MLP -> queue order + tile -> sift launch.
+            # The plan runs on its own stream: it reads only the input, so it need not wait
+            # for the previous step's sift, and its one host read (the tile size) waits for
+            # the plan alone.  Its kernels take CU slots as the previous sift's workgroups
+            # retire.
+            main = torch.cuda.current_stream()
+            if plan_stream is None:
+                f = costmodel.features(dx.view(-1), doff)
+                plan = cm.plan(f, dlen, nbsym=cfg.nbsym)
+            else:
+                with torch.cuda.stream(plan_stream):
+                    f = costmodel.features(dx.view(-1), doff)
+                    plan = cm.plan(f, dlen, nbsym=cfg.nbsym)
+                main.wait_stream(plan_stream)
+                plan.order.record_stream(main)
             tile = args.tile or plan.tile_slots
             plan_tile.append(tile)
             return sift_call(lambda: launcher.run_ragged(dx.view(-1), offsets, order=plan.order,
