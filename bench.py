@@ -433,6 +433,9 @@ def main():
     ap.add_argument("--tile", type=int, default=0,
                     help="LDS knot slots of the packed path (0: sized by the cost model)")
     ap.add_argument("--threads", type=int, default=256)
+    ap.add_argument("--serial-steps", action="store_true",
+                    help="packed path: each step's sift starts after the previous one ends "
+                         "(default: consecutive steps overlap on two handles and streams)")
     ap.add_argument("--serial-plan", action="store_true",
                     help="packed path: compute each step's plan on the launch stream, after "
                          "the previous step's sift (default: on a planning stream)")
@@ -505,17 +508,20 @@ def main():
         events.append((e0, e1))
         return out
 
-    def timed(fn):
+    def timed(fn, pipe=None):
         for _ in range(args.warmup):
             fn()
         torch.cuda.synchronize()
         events.clear()
+        if pipe is not None:
+            pipe.launch_ms()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             out = fn()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        kernel_ms = [a.elapsed_time(b) for a, b in events]
+        kernel_ms = (pipe.launch_ms() if pipe is not None
+                     else [a.elapsed_time(b) for a, b in events])
         return out, wall / args.steps * 1e3, float(np.mean(kernel_ms))
 
     if args.mode in ("both", "plain"):
@@ -534,6 +540,7 @@ def main():
     model_info = None
     if args.mode in ("both", "packed"):
         from cuda_emd import costmodel
+        from cuda_emd.packed import PipelinedEMD
         cm, model_info = train_cost_model(launcher, args)
         doff = offsets.cuda()
         plan_tile = []
@@ -563,7 +570,20 @@ def main():
             return sift_call(lambda: launcher.run_ragged(dx.view(-1), offsets, order=plan.order,
                                                          threads=args.threads, tile_slots=tile,
                                                          persistent=True))
-        out, ms_wall, ms_ev = timed(packed)
+
+        pipe = None
+        if not args.serial_steps:
+            # Consecutive steps overlap: each step's sift runs on the next of two lanes
+            # (handle + stream) and takes the CU slots the previous step's sift frees as its
+            # queue drains.  Every step still plans and decomposes its whole batch.
+            pipe = PipelinedEMD(cfg, cm, threads=args.threads, tile_slots=args.tile,
+                                timing=True)
+
+            def packed():  # noqa: F811
+                out = pipe.submit(dx.view(-1), offsets, offsets_dev=doff, lengths_dev=dlen)
+                plan_tile.append(pipe.last_tile)
+                return out
+        out, ms_wall, ms_ev = timed(packed, pipe)
         results["packed"] = dict(ms=ms_wall, kernel_ms=ms_ev, n_imfs=out[2].cpu().numpy(),
                                  stats=out[7].cpu().numpy(), out=out)
         model_info["tile_slots"] = plan_tile[-1]
@@ -590,7 +610,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "seismic traces, 16384 x N=2001, seed 1", "traces": B, "N": N,
-                   "path": head, "max_imfs": cfg.max_imfs, "max_extrema": cfg.max_extrema},
+                   "path": head, "max_imfs": cfg.max_imfs, "max_extrema": cfg.max_extrema,
+                   "steps_overlap": (head == "packed" and not args.serial_steps)},
         # the compute roof: MI355X's fp64 peak (78.6 TFLOP/s, the same for its vector ALUs,
         # which run the sift, as for its matrix cores, which it does not use)
         "roofline": {"bound": "mfma", "roof": "fp64 arithmetic (VALU)", "achieved": achieved,

@@ -273,3 +273,99 @@ class PackedEMD:
             self.close()
         except Exception:
             pass
+
+
+class PipelinedEMD:
+    """A stream of batches through the learned-packed persistent kernel, with consecutive
+    batches overlapped.
+
+    Each ``submit`` plans its batch (features -> MLP -> queue order + LDS tile; the FIFO
+    order and ``tile_slots`` without a cost model) on a planning stream, then launches the
+    persistent sift on the next of ``depth`` lanes.  A lane is a launcher handle (its own
+    queue counter and global knot tiles) with its own stream, so a batch's sift takes the
+    CU slots the previous batch's sift frees as its queue drains, instead of leaving them
+    idle in that launch's tail.  The plan reads only its batch, so it also runs while the
+    previous sift is still in flight.
+
+    ``submit`` returns run_ragged's tuple without waiting; the results are ready once
+    ``done(result)`` has been waited on (``torch.cuda.current_stream().wait_event``) or
+    after ``torch.cuda.synchronize()``.  A batch's input must stay unchanged until then.
+    Outputs are those of one run_ragged call per batch: the lanes run the same kernel.
+    With ``timing``, HIP events bracket every sift launch on its lane's stream
+    (``launch_ms()`` reads them).
+    """
+
+    def __init__(self, config: EMDKernelConfig | None = None, cost_model=None,
+                 threads: int = 256, tile_slots: int = 0, depth: int = 2,
+                 timing: bool = False):
+        torch = _torch()
+        if depth < 1:
+            raise ValueError(f"depth must be >= 1, got {depth}")
+        self.config = config or EMDKernelConfig()
+        self.cost_model = cost_model
+        self.threads, self.tile_slots = threads, tile_slots
+        self.lanes = [(self.config.build(), torch.cuda.Stream()) for _ in range(depth)]
+        self.plan_stream = torch.cuda.Stream()
+        self.timing = timing
+        self._events = []
+        self._done = {}
+        self._k = 0
+        self.last_tile = tile_slots
+
+    def _plan(self, values, offsets_dev, lengths_dev):
+        from . import costmodel
+        cm = self.cost_model
+        if cm is None:
+            return None, self.tile_slots
+        f = costmodel.features(values, offsets_dev)
+        plan = cm.plan(f, lengths_dev, nbsym=self.config.nbsym, quantile=0.995)
+        return plan.order, self.tile_slots or plan.tile_slots
+
+    def submit(self, values, offsets, offsets_dev=None, lengths_dev=None) -> tuple:
+        """Queue one CSR batch (float64 device ``values``, int64 ``offsets`` [B+1]).
+        ``offsets_dev`` / ``lengths_dev`` (device int64 offsets, float64 lengths) spare the
+        per-call host-to-device copies when the caller already holds them."""
+        torch = _torch()
+        if offsets_dev is None:
+            offsets_dev = (offsets if isinstance(offsets, torch.Tensor)
+                           else torch.from_numpy(np.asarray(offsets, np.int64))).cuda()
+        if lengths_dev is None:
+            lengths_dev = torch.diff(offsets_dev).double()
+        with torch.cuda.stream(self.plan_stream):
+            order, tile = self._plan(values, offsets_dev, lengths_dev)
+        self.last_tile = tile
+        launcher, st = self.lanes[self._k % len(self.lanes)]
+        self._k += 1
+        st.wait_stream(self.plan_stream)
+        st.wait_stream(torch.cuda.current_stream())   # the caller's writes of the input
+        if order is not None:
+            order.record_stream(st)
+        with torch.cuda.stream(st):
+            if self.timing:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e0.record(st)
+            out = launcher.run_ragged(values, offsets, order=order, threads=self.threads,
+                                      tile_slots=tile, persistent=True)
+            done = torch.cuda.Event(enable_timing=self.timing)
+            done.record(st)
+            if self.timing:
+                self._events.append((e0, done))
+        for t in (values, offsets_dev, lengths_dev):
+            if isinstance(t, torch.Tensor) and t.is_cuda:
+                t.record_stream(st)
+                t.record_stream(self.plan_stream)
+        self._done[id(out[0])] = done
+        if len(self._done) > 4 * len(self.lanes):
+            self._done.pop(next(iter(self._done)))
+        return out
+
+    def done(self, result):
+        """The event recorded after ``result``'s sift."""
+        return self._done[id(result[0])]
+
+    def launch_ms(self, clear: bool = True) -> list:
+        """Durations (ms) of the timed sift launches so far (waits for them)."""
+        ms = [a.elapsed_time(b) for a, b in self._events]
+        if clear:
+            self._events.clear()
+        return ms

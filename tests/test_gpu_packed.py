@@ -7,7 +7,7 @@ import torch
 
 from cuda_emd import EMDKernelConfig, costmodel, synth
 from cuda_emd.kernels import ChunkGraph
-from cuda_emd.packed import Autotuner, PackedEMD
+from cuda_emd.packed import Autotuner, PackedEMD, PipelinedEMD
 from cuda_emd.stream import StreamingEMD
 
 pytestmark = pytest.mark.gpu
@@ -110,3 +110,31 @@ def test_packed_ragged_matches_plain(tmp_path, learned):
         _eager(launcher, values[:offsets[1]], offsets[:2])  # eager work between replays
     assert (tmp_path / "tune.json").exists()
     p.close()
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_pipelined_batches_match_eager(learned):
+    """Consecutive batches overlapped on two lanes (handle + stream each): every batch's
+    outputs equal one eager launch over it, also when two batches' sifts run at once."""
+    cfg = EMDKernelConfig()
+    launcher = cfg.build()
+    batches = [synth.ragged_mix(300, seed=60 + i, n_lo=64, n_hi=3000)[:2] for i in range(3)]
+    refs = [_eager(launcher, v, o) for v, o in batches]
+    cm = None
+    if learned:
+        tv, to, _ = synth.ragged_mix(400, seed=59, n_lo=64, n_hi=3000)
+        tr = _eager(launcher, tv, to, persistent=True)
+        cm = costmodel.CostModel(seed=3)
+        cm.fit(costmodel.features(torch.from_numpy(tv).cuda(), torch.from_numpy(to).cuda()),
+               costmodel.targets_from(tr[7], tr[2]), epochs=5, batch=128)
+    pipe = PipelinedEMD(cfg, cost_model=cm, depth=2, timing=True)
+    dev = [(torch.from_numpy(v).cuda(), torch.from_numpy(o)) for v, o in batches]
+    print(f"pipelined learned={learned}", flush=True)
+    outs = []
+    for rnd in range(2):
+        for i, (v, o) in enumerate(dev):
+            outs.append((i, pipe.submit(v, o)))
+    torch.cuda.synchronize()
+    assert len(pipe.launch_ms()) == 6
+    for i, got in outs:
+        _assert_same(refs[i], got, batches[i][1], f"pipelined batch {i}")
