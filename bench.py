@@ -433,6 +433,8 @@ def main():
     ap.add_argument("--tile", type=int, default=0,
                     help="LDS knot slots of the packed path (0: sized by the cost model)")
     ap.add_argument("--threads", type=int, default=256)
+    ap.add_argument("--lanes", type=int, default=2,
+                    help="packed path: handles/streams consecutive steps rotate over")
     ap.add_argument("--serial-steps", action="store_true",
                     help="packed path: each step's sift starts after the previous one ends "
                          "(default: consecutive steps overlap on two handles and streams)")
@@ -577,7 +579,7 @@ def main():
             # (handle + stream) and takes the CU slots the previous step's sift frees as its
             # queue drains.  Every step still plans and decomposes its whole batch.
             pipe = PipelinedEMD(cfg, cm, threads=args.threads, tile_slots=args.tile,
-                                timing=True)
+                                depth=args.lanes, timing=True)
 
             def packed():  # noqa: F811
                 out = pipe.submit(dx.view(-1), offsets, offsets_dev=doff, lengths_dev=dlen)
