@@ -384,6 +384,9 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
   }
   // ---- exclusive prefix of per-block counts (wave 0; up to 4 blocks per lane) ---------
   if (w == 0) {
+    // One wave works while the workgroup waits at the barrier below; raised issue
+    // priority lets it go ahead of the other workgroups' waves sharing its SIMD.
+    __builtin_amdgcn_s_setprio(2);
     const int per = ceil_div(nblk, 64);
     unsigned loc = 0;
     for (int k = 0; k < per; ++k) {
@@ -404,6 +407,7 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
       sh.tot_max = int(incl & 0xffffu);
       sh.tot_min = int(incl >> 16);
     }
+    __builtin_amdgcn_s_setprio(0);
   }
   __syncthreads();
   DetectResult R;
@@ -944,9 +948,12 @@ __device__ __forceinline__ void spline_solve(const Tile& t, const SiftShared<WG>
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
       const int owner = WG >= 128 ? side : 0;
-      if (wv == owner && uni(sh.kind[side]) == kNotAKnot)
+      if (wv == owner && uni(sh.kind[side]) == kNotAKnot) {
+        __builtin_amdgcn_s_setprio(2);  // one wave per side; the others wait at the barrier
         solve_side(t, side, uni(sh.k0[side]), uni(sh.nk[side]), side ? 3 * nb + D.nm : nb,
                    side ? D.nn : D.nm, lane);
+        __builtin_amdgcn_s_setprio(0);
+      }
     }
   }
   __syncthreads();
@@ -1212,6 +1219,7 @@ __device__ __forceinline__ void build_envelopes(const Tile& t, SiftShared<WG>& s
                                                 const SiftParams& P, const Work& wk) {
   constexpr int NW = WG / 64;
   if (threadIdx.x < 64) {
+    __builtin_amdgcn_s_setprio(2);  // the mirror is the longest single-wave step
     int bad = sh.bad;
     int k0[2], nk[2], kind[2];
     double p3[2][4];
@@ -1228,6 +1236,7 @@ __device__ __forceinline__ void build_envelopes(const Tile& t, SiftShared<WG>& s
       const int m1 = kind[1] == kNotAKnot ? nk[1] - 2 : 0;
       sh.mtot = m0 + m1;
     }
+    __builtin_amdgcn_s_setprio(0);
   }
   // interval setup between consecutive true knots of both sides (the mirror does not move
   // them): waves 1.. while wave 0 mirrors; wave 0 itself when it is the only one
