@@ -26,7 +26,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from .kernels import EMDBatchLauncher, EMDKernelConfig, _ptr, _stream
+from .kernels import EMDBatchLauncher, EMDKernelConfig, _as_device_f64, _ptr, _stream
 
 BUCKET_BOUNDS = (512, 1024, 2048, 4096, 8192, 16384)
 
@@ -326,6 +326,7 @@ class PipelinedEMD:
         ``offsets_dev`` / ``lengths_dev`` (device int64 offsets, float64 lengths) spare the
         per-call host-to-device copies when the caller already holds them."""
         torch = _torch()
+        values = _as_device_f64(values, "values")   # float64 only, as run_ragged
         if offsets_dev is None:
             offsets_dev = (offsets if isinstance(offsets, torch.Tensor)
                            else torch.from_numpy(np.asarray(offsets, np.int64))).cuda()
