@@ -153,6 +153,9 @@ def decode_segy(dev_bytes, layout: SegyLayout, out=None):
                                        TRACE_HEADER_BYTES, None, N, B, layout.format,
                                        _ptr(out), _stream()), "emd_segy_decode")
         return out
+    if out is not None:
+        raise ValueError("out is for fixed-length layouts; a ragged layout allocates its "
+                         "CSR values")
     offsets_h = layout.offsets
     if int(layout.lengths.max(initial=0)) >= 2 ** 31:
         raise ValueError("a trace longer than 2^31 - 1 samples")
