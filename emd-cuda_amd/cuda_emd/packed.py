@@ -242,13 +242,22 @@ class PackedEMD:
         for b, idx in self.members.items():
             self.order_bufs[b].copy_(_torch().from_numpy(self._order(idx, pred).astype(np.int32)))
 
-    def run(self, refresh: bool = True):
+    def run(self, refresh: bool = True, stream=None):
         """Decompose the prepared batch; returns run_ragged's tuple (imfs, out_offsets,
-        n_imfs, overflow, debug_n, debug_extno, debug_nzm, stats)."""
+        n_imfs, overflow, debug_n, debug_extno, debug_nzm, stats).  The bucket launches
+        fork from and join back to ``stream`` (default: the current stream).  The queue
+        order is refreshed on a planning stream once this batch's previous run has joined,
+        so two prepared batches run from two streams overlap each other's plans and
+        tails."""
         torch = _torch()
-        if refresh:
-            self.refresh_order()
-        main = torch.cuda.current_stream()
+        main = stream if stream is not None else torch.cuda.current_stream()
+        if refresh and self.cost_model is not None:
+            if getattr(self, "plan_stream", None) is None:
+                self.plan_stream = torch.cuda.Stream()
+            self.plan_stream.wait_stream(main)   # the order buffers are free again
+            with torch.cuda.stream(self.plan_stream):
+                self.refresh_order()
+            main.wait_stream(self.plan_stream)
         start = torch.cuda.Event()
         start.record(main)
         for b in self.launch_order:

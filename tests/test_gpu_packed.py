@@ -138,3 +138,29 @@ def test_pipelined_batches_match_eager(learned):
     assert len(pipe.launch_ms()) == 6
     for i, got in outs:
         _assert_same(refs[i], got, batches[i][1], f"pipelined batch {i}")
+
+
+def test_packed_instances_overlapped_on_two_streams(tmp_path):
+    """Two prepared instances of a batch run alternately from two streams (plans on their
+    own streams): every run's outputs equal the eager launch's."""
+    values, offsets, _ = synth.ragged_mix(500, seed=70, n_lo=64, n_hi=4000)
+    cfg = EMDKernelConfig()
+    launcher = cfg.build()
+    ref = _eager(launcher, values, offsets)
+    tv, to, _ = synth.ragged_mix(400, seed=71, n_lo=64, n_hi=4000)
+    tr = _eager(launcher, tv, to, persistent=True)
+    cm = costmodel.CostModel(seed=4)
+    cm.fit(costmodel.features(torch.from_numpy(tv).cuda(), torch.from_numpy(to).cuda()),
+           costmodel.targets_from(tr[7], tr[2]), epochs=5, batch=128)
+    tuner = Autotuner(cfg, cache_path=str(tmp_path / "tune.json"), sample=128)
+    inst = [PackedEMD(cfg, cost_model=cm, autotuner=tuner).prepare(values, offsets)
+            for _ in range(2)]
+    homes = [torch.cuda.Stream(), torch.cuda.Stream()]
+    print("packed instances on two streams", flush=True)
+    for rnd in range(2):
+        outs = [inst[i].run(stream=homes[i]) for i in range(2)]
+        torch.cuda.synchronize()
+        for i, got in enumerate(outs):
+            _assert_same(ref, got, offsets, f"instance {i} round {rnd}")
+    for p in inst:
+        p.close()
