@@ -129,3 +129,12 @@ def test_iter_segy_chunks_match_whole_read(tmp_path, fmt, ragged):
                                   _bits(whole.signals.cpu().numpy()[t0:t0 + n]))
         seen += n
     assert seen == len(lengths)
+
+
+def test_read_segy_without_traces(tmp_path):
+    """A file holding only its 3600-byte header reads as an empty [0, ns] batch."""
+    data = segy_files.segy_bytes([], [], 1, ns=250)
+    print("segy header only", flush=True)
+    tr = _read(tmp_path, data, "empty.sgy")
+    assert tr.layout.fixed and tuple(tr.signals.shape) == (0, 250)
+    assert list(traceio.iter_segy(str(tmp_path / "empty.sgy"))) == []
