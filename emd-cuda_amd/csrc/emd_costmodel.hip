@@ -259,18 +259,68 @@ __device__ __forceinline__ void forward_tile(MlpShared& sh) {
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kThreads) void mlp_forward_kernel(const float* __restrict__ params,
-                                                              const float* __restrict__ norm,
-                                                              const float* __restrict__ feats,
-                                                              int M, float* __restrict__ out) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  MlpShared& sh = *reinterpret_cast<MlpShared*>(smem);
-  load_tile(sh, params, norm, feats, M);
-  forward_tile(sh);
-  const int t = threadIdx.x, s0 = blockIdx.x * kTile;
-  if (t < kO * kTile) {
+// Inference only, with a small LDS footprint (about 9 KB): 16 samples per workgroup, the
+// weights read through the caches instead of staged in LDS.  The plan's forward pass runs
+// while a persistent sift still holds most of every CU's LDS, so a small workgroup starts
+// as soon as one sift workgroup retires.  Every sum runs in the same order as in
+// forward_tile, so the outputs are the same bits.
+constexpr int kTileF = 16;
+constexpr int kPerF = kTileF / (kThreads / kH);  // 4 samples per lane
+
+__global__ __launch_bounds__(kThreads) void mlp_infer_kernel(const float* __restrict__ params,
+                                                            const float* __restrict__ norm,
+                                                            const float* __restrict__ feats,
+                                                            int M, float* __restrict__ out) {
+  __shared__ float x[kTileF][kF + 1];
+  __shared__ float h1[kTileF][kLd], h2[kTileF][kLd];
+  __shared__ float y[kTileF][kO];
+  const int t = threadIdx.x, j = t & (kH - 1), s0 = (t >> 6) * kPerF;
+  const int base = blockIdx.x * kTileF;
+  for (int e = t; e < kTileF * kF; e += kThreads) {
+    const int s = e / kF, k = e % kF;
+    const float f = base + s < M ? feats[(int64_t)(base + s) * kF + k] : 0.0f;
+    x[s][k] = (f - norm[k]) / norm[kF + k];
+  }
+  __syncthreads();
+  {
+    float w[kF];
+#pragma unroll
+    for (int k = 0; k < kF; ++k) w[k] = params[kW1 + j * kF + k];
+    const float b = params[kB1 + j];
+#pragma unroll
+    for (int q = 0; q < kPerF; ++q) {
+      float z = b;
+#pragma unroll
+      for (int k = 0; k < kF; ++k) z += w[k] * x[s0 + q][k];
+      h1[s0 + q][j] = gelu(z);
+    }
+  }
+  __syncthreads();
+  {
+    float acc[kPerF];
+    const float b = params[kB2 + j];
+#pragma unroll
+    for (int q = 0; q < kPerF; ++q) acc[q] = b;
+    const float* w2 = params + kW2 + j * kH;  // W2[j][k]
+    for (int k = 0; k < kH; ++k) {
+      const float w = w2[k];
+#pragma unroll
+      for (int q = 0; q < kPerF; ++q) acc[q] += w * h1[s0 + q][k];
+    }
+#pragma unroll
+    for (int q = 0; q < kPerF; ++q) h2[s0 + q][j] = gelu(acc[q]);
+  }
+  __syncthreads();
+  if (t < kO * kTileF) {
+    const int o = t / kTileF, s = t % kTileF;
+    float v = params[kB3 + o];
+    for (int k = 0; k < kH; ++k) v += params[kW3 + o * kH + k] * h2[s][k];
+    y[s][o] = v;
+  }
+  __syncthreads();
+  if (t < kO * kTileF) {
     const int s = t / kO, o = t % kO;
-    if (s0 + s < M) out[(int64_t)(s0 + s) * kO + o] = sh.y[s][o];
+    if (base + s < M) out[(int64_t)(base + s) * kO + o] = y[s][o];
   }
 }
 
@@ -435,11 +485,9 @@ int emd_mlp_forward(const float* params, const float* norm, const float* feats, 
   if (M < 0) return fail(EMD_ERR_VALUE, "sample count must be >= 0");
   if (M == 0) return EMD_OK;
   if (!params || !norm || !feats || !out) return fail(EMD_ERR_VALUE, "null pointer");
-  int rc = mlp_smem_attr(reinterpret_cast<const void*>(&emdcm::mlp_forward_kernel));
-  if (rc) return rc;
-  const int nwg = (M + emdcm::kTile - 1) / emdcm::kTile;
-  hipLaunchKernelGGL(emdcm::mlp_forward_kernel, dim3(nwg), dim3(emdcm::kThreads),
-                     sizeof(emdcm::MlpShared), (hipStream_t)stream, params, norm, feats, M, out);
+  const int nwg = (M + emdcm::kTileF - 1) / emdcm::kTileF;
+  hipLaunchKernelGGL(emdcm::mlp_infer_kernel, dim3(nwg), dim3(emdcm::kThreads), 0,
+                     (hipStream_t)stream, params, norm, feats, M, out);
   return launch_status("mlp forward kernel");
 }
 
