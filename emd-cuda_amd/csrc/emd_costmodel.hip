@@ -66,9 +66,11 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__global__ __launch_bounds__(256) void features_kernel(const double* __restrict__ values,
-                                                        const int64_t* __restrict__ offsets,
-                                                        float* __restrict__ feats) {
+// waves_per_eu(5) holds the kernel to 96 VGPRs (94 used, no spills): 5 workgroups per CU
+// instead of the 4 its 100 unconstrained VGPRs allowed (0.40 -> 0.36 ms for 16384 x 2001).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void features_kernel(
+    const double* __restrict__ values, const int64_t* __restrict__ offsets,
+    float* __restrict__ feats) {
   // each wave transforms its own segments: radix-2 FFT of 256 points in LDS, fp64
   __shared__ double re[4][kSeg], im[4][kSeg];
   __shared__ double cs[kSeg / 2], sn[kSeg / 2];
