@@ -341,13 +341,15 @@ class PipelinedEMD:
                            else torch.from_numpy(np.asarray(offsets, np.int64))).cuda()
         if lengths_dev is None:
             lengths_dev = torch.diff(offsets_dev).double()
+        caller = torch.cuda.current_stream()
+        self.plan_stream.wait_stream(caller)          # the caller's writes of the input
         with torch.cuda.stream(self.plan_stream):
             order, tile = self._plan(values, offsets_dev, lengths_dev)
         self.last_tile = tile
         launcher, st = self.lanes[self._k % len(self.lanes)]
         self._k += 1
         st.wait_stream(self.plan_stream)
-        st.wait_stream(torch.cuda.current_stream())   # the caller's writes of the input
+        st.wait_stream(caller)
         if order is not None:
             order.record_stream(st)
         with torch.cuda.stream(st):
