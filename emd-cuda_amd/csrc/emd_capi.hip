@@ -53,6 +53,12 @@ struct emd_handle {
   int timing = 0;                // record ev[0]/ev[1] around the sift launches
   int timed = 0;                 // the last run recorded them
   hipEvent_t ev[2] = {nullptr, nullptr};
+  // The scratch above is shared by every launch of the handle.  `busy` is recorded after
+  // each launch on its stream; a launch on another stream first waits for it, so launches
+  // of one handle never overlap each other's queue counter or global knot tiles.
+  hipEvent_t busy = nullptr;
+  hipStream_t busy_stream = nullptr;
+  int busy_set = 0;
 };
 
 namespace {
@@ -220,6 +226,12 @@ int emd_create(const emd_config* cfg, emd_handle** out) {
     delete h;
     return fail(EMD_ERR_MEMORY, "hipMalloc(counters): %s", hipGetErrorString(e));
   }
+  e = hipEventCreateWithFlags(&h->busy, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    (void)hipFree(h->counters);
+    delete h;
+    return hip_fail(e, "hipEventCreateWithFlags");
+  }
   *out = h;
   return EMD_OK;
 }
@@ -229,6 +241,7 @@ int emd_destroy(emd_handle* h) {
   hipError_t e = hipDeviceSynchronize();
   for (hipEvent_t ev : h->ev)
     if (ev) (void)hipEventDestroy(ev);
+  if (h->busy) (void)hipEventDestroy(h->busy);
   if (h->spill) (void)hipFree(h->spill);
   if (h->counters) (void)hipFree(h->counters);
   delete h;
@@ -310,6 +323,33 @@ static int enqueue(emd_handle* h, const Plan& pl, emdk::Batch b, int B, char* sp
   return timed ? mark(h, 1, st) : EMD_OK;
 }
 
+// Launches that use the handle's scratch are ordered: one on a stream other than the
+// previous launch's first waits for that launch (skipped under stream capture, where the
+// caller's graph orders its own nodes).
+static int scratch_acquire(emd_handle* h, hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hipError_t e = hipStreamIsCapturing(st, &cs);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamIsCapturing");
+  if (cs != hipStreamCaptureStatusNone) return EMD_OK;
+  if (h->busy_set && h->busy_stream != st) {
+    e = hipStreamWaitEvent(st, h->busy, 0);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamWaitEvent(previous launch)");
+  }
+  return EMD_OK;
+}
+
+static int scratch_release(emd_handle* h, hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hipError_t e = hipStreamIsCapturing(st, &cs);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamIsCapturing");
+  if (cs != hipStreamCaptureStatusNone) return EMD_OK;
+  e = hipEventRecord(h->busy, st);
+  if (e != hipSuccess) return hip_fail(e, "hipEventRecord(launch done)");
+  h->busy_stream = st;
+  h->busy_set = 1;
+  return EMD_OK;
+}
+
 static int check_io(int B, int max_n, const double* values, double* imfs, int* n_imfs,
                     int* overflow, int* debug_n, int* debug_extno, int* debug_nzm) {
   if (B < 0) return fail(EMD_ERR_VALUE, "batch size must be >= 0, got %d", B);
@@ -321,14 +361,15 @@ static int check_io(int B, int max_n, const double* values, double* imfs, int* n
 }
 
 static emdk::Batch make_batch(const double* values, const int64_t* offsets, int N,
-                              const int* order, double* imfs, const int64_t* out_offsets,
-                              int* n_imfs, int* overflow, int* debug_n, int* debug_extno,
-                              int* debug_nzm, int* stats) {
+                              const int* order, int total, double* imfs,
+                              const int64_t* out_offsets, int* n_imfs, int* overflow,
+                              int* debug_n, int* debug_extno, int* debug_nzm, int* stats) {
   emdk::Batch b;
   b.values = values;
   b.offsets = offsets;
   b.N = N;
   b.order = order;
+  b.total = total;
   b.imfs = imfs;
   b.out_offsets = out_offsets;
   b.n_imfs = n_imfs;
@@ -348,12 +389,17 @@ static int run_common(emd_handle* h, const double* values, const int64_t* offset
   if (rc) return rc;
   h->timed = 0;
   if (B == 0) return EMD_OK;
+  if (L.traces < 0) return fail(EMD_ERR_VALUE, "launch.traces must be >= 0, got %d", L.traces);
   Plan pl;
   if ((rc = make_plan(h, B, max_n, L, &pl))) return rc;
   if (pl.spill_bytes && (rc = ensure_spill(h, pl.spill_bytes))) return rc;
-  const emdk::Batch b = make_batch(values, offsets, N, order, imfs, out_offsets, n_imfs,
-                                   overflow, debug_n, debug_extno, debug_nzm, stats);
-  return enqueue(h, pl, b, B, h->spill, h->counters, h->timing != 0, st);
+  const emdk::Batch b = make_batch(values, offsets, N, order, L.traces > 0 ? L.traces : B,
+                                   imfs, out_offsets, n_imfs, overflow, debug_n, debug_extno,
+                                   debug_nzm, stats);
+  if ((rc = scratch_acquire(h, st))) return rc;
+  rc = enqueue(h, pl, b, B, h->spill, h->counters, h->timing != 0, st);
+  const int rc2 = scratch_release(h, st);
+  return rc ? rc : rc2;
 }
 
 int emd_run_padded(emd_handle* h, const double* signals, int B, int N, double* imfs,
@@ -367,6 +413,7 @@ int emd_run_padded(emd_handle* h, const double* signals, int B, int N, double* i
   L.tile_slots = h->cfg.tile_slots;
   L.wgs_per_cu = 0;
   L.persistent = 0;
+  L.traces = 0;
   return run_common(h, signals, nullptr, N, B, N, nullptr, L, imfs, nullptr, n_imfs, overflow,
                     debug_n, debug_extno, debug_nzm, stats, (hipStream_t)stream);
 }
@@ -387,6 +434,7 @@ int emd_run_ragged(emd_handle* h, const double* values, const int64_t* offsets, 
     L.tile_slots = h->cfg.tile_slots;
     L.wgs_per_cu = 0;
     L.persistent = 1;
+    L.traces = 0;
   }
   return run_common(h, values, offsets, 0, B, max_n, order, L, imfs, out_offsets, n_imfs,
                     overflow, debug_n, debug_extno, debug_nzm, stats, (hipStream_t)stream);
@@ -411,6 +459,7 @@ int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double
     rc = ensure_spill(h, (size_t)stride * std::min(B, kMaxPlainBlocks));
     if (rc) return rc;
   }
+  if ((rc = scratch_acquire(h, (hipStream_t)stream))) return rc;
   for (int base = 0; base < B; base += kMaxPlainBlocks) {
     const int nb = std::min(kMaxPlainBlocks, B - base);
     emdk::EnvBatch e;
@@ -434,7 +483,7 @@ int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double
     rc = check_launch("envelope kernel");
     if (rc) return rc;
   }
-  return EMD_OK;
+  return scratch_release(h, (hipStream_t)stream);
 }
 
 }  // extern "C"
@@ -469,6 +518,8 @@ int emd_chunk_graph_create(emd_handle* h, const emd_chunk* c, emd_graph** out) {
   if (c->B < 1) return fail(EMD_ERR_VALUE, "a captured chunk needs B >= 1");
   if (!c->row_offsets != !c->compact)
     return fail(EMD_ERR_VALUE, "compaction needs both row_offsets and compact");
+  if (c->launch.traces < 0)
+    return fail(EMD_ERR_VALUE, "launch.traces must be >= 0, got %d", c->launch.traces);
   Plan pl;
   if ((rc = make_plan(h, c->B, max_n, c->launch, &pl))) return rc;
   emd_graph* g = new emd_graph();
@@ -491,7 +542,8 @@ int emd_chunk_graph_create(emd_handle* h, const emd_chunk* c, emd_graph** out) {
     graph_free(g);
     return hip_fail(e, "hipStreamBeginCapture");
   }
-  const emdk::Batch b = make_batch(c->values, c->offsets, c->N, c->order, c->imfs,
+  const emdk::Batch b = make_batch(c->values, c->offsets, c->N, c->order,
+                                   c->launch.traces > 0 ? c->launch.traces : c->B, c->imfs,
                                    c->out_offsets, c->n_imfs, c->overflow, c->debug_n,
                                    c->debug_extno, c->debug_nzm, c->stats);
   rc = enqueue(h, pl, b, c->B, g->spill, g->counters, false, cs);

@@ -33,6 +33,8 @@ struct Batch {
   int B;                       // traces handled by this launch
   int base;                    // first queue position of this launch
   const int* order;            // processing order or null (FIFO)
+  int total;                   // traces the offsets describe: order entries outside
+                               // [0, total) are skipped
   double* imfs;
   const int64_t* out_offsets;  // per-signal output base (elements) or null (padded)
   int* n_imfs;

@@ -1613,6 +1613,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void em
   const WgMem m = workgroup_mem(dyn_lds, b.spill, b.spill_stride, P);
   const int q = b.base + (int)blockIdx.x;
   const int s = b.order ? b.order[q] : q;
+  if ((unsigned)s >= (unsigned)b.total) return;  // a queue entry naming no trace
   decompose<WG>(signal_io(b, s, P.max_imfs), sh, m.wk, m.tl, m.tg, P);
 }
 
@@ -1631,6 +1632,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void em
     if (idx >= b.B) break;
     const int q = b.base + idx;
     const int s = b.order ? b.order[q] : q;
+    if ((unsigned)s >= (unsigned)b.total) continue;  // a queue entry naming no trace
     decompose<WG>(signal_io(b, s, P.max_imfs), sh, m.wk, m.tl, m.tg, P);
   }
 }

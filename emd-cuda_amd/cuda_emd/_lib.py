@@ -28,7 +28,8 @@ class EmdConfigC(ctypes.Structure):
 
 class EmdLaunchC(ctypes.Structure):
     _fields_ = [("threads", ctypes.c_int), ("tile_slots", ctypes.c_int),
-                ("wgs_per_cu", ctypes.c_int), ("persistent", ctypes.c_int)]
+                ("wgs_per_cu", ctypes.c_int), ("persistent", ctypes.c_int),
+                ("traces", ctypes.c_int)]
 
 
 class EmdChunkC(ctypes.Structure):

@@ -119,6 +119,27 @@ def _as_device_f64(x, what: str):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
+def warn_nonuniform_T(T) -> bool:
+    """Warn (UserWarning) when T is not uniformly spaced; returns True if it warned.
+
+    PyEMD's EMD() in its default 'simple' extrema mode replaces T by the sample index, and
+    so does this build: a non-uniform T gives the decomposition of T = arange(N), not a
+    time-warped one (the reference instead normalises T and uses it in its boundary mirror
+    and spline evaluation, kernels.py:471-472, 528, 703; emd.py:83-88)."""
+    import warnings
+    T = np.asarray(T, dtype=np.float64)
+    if T.size < 3:
+        return False
+    d = np.diff(T)
+    if np.all(d > 0) and np.max(np.abs(d / d[0] - 1.0)) <= 1e-12:
+        return False
+    warnings.warn(
+        "T is not uniformly spaced: like PyEMD's EMD() in its default 'simple' extrema "
+        "mode, this build uses the sample index as the time axis, so the result is the "
+        "decomposition for T = arange(N).", UserWarning, stacklevel=3)
+    return True
+
+
 def _ptr(t) -> ctypes.c_void_p:
     return ctypes.c_void_p(t.data_ptr() if t is not None and t.numel() > 0 else 0)
 
@@ -167,6 +188,7 @@ class EMDBatchLauncher:
                 f"T.dtype={Tn.dtype}.")
         if Tn.shape != (N,):
             raise ValueError(f"T must have shape ({N},), got {Tn.shape}")
+        warn_nonuniform_T(Tn)
 
     def run(self, signals, T=None) -> tuple:
         """Decompose ``signals`` [B,N] float64.  Returns (imfs [B,max_imfs,N], n_imfs [B],
@@ -212,8 +234,9 @@ class EMDBatchLauncher:
         if B < 0:
             raise ValueError("offsets needs at least one entry")
         lengths = np.diff(off_host)
-        if B and (lengths.min() < 0 or off_host[-1] > v.numel()):
-            raise ValueError("offsets must be non-decreasing and within values")
+        if B and (off_host[0] < 0 or lengths.min() < 0 or off_host[-1] > v.numel()):
+            raise ValueError("offsets must start at >= 0, be non-decreasing and stay "
+                             "within values")
         max_n = int(lengths.max()) if B else 0
         c = self.config
         dev = v.device
@@ -230,12 +253,23 @@ class EMDBatchLauncher:
         dn, de, dz = (torch.empty((B, c.max_imfs), **i32) for _ in range(3))
         stats = torch.empty((B, 4), **i32)
         if order is not None:
+            if not (isinstance(order, torch.Tensor) and order.is_cuda):
+                # a host order is checked here: a permutation of the B traces (an entry
+                # out of range would be skipped by the kernel, a repeated one would leave
+                # another trace's outputs unwritten).  A device order is the caller's
+                # promise; the kernel still skips entries outside [0, B).
+                oh = np.asarray(order.numpy() if isinstance(order, torch.Tensor) else order)
+                if oh.shape != (B,):
+                    raise ValueError(f"order must have {B} entries, got shape {oh.shape}")
+                if B and (oh.min() < 0 or oh.max() >= B
+                          or not np.all(np.bincount(oh.astype(np.int64), minlength=B) == 1)):
+                    raise ValueError("order must be a permutation of range(B)")
             order = (order if isinstance(order, torch.Tensor) else torch.from_numpy(
                 np.ascontiguousarray(np.asarray(order, dtype=np.int32)))).to(dev).int().contiguous()
             if order.numel() != B:
                 raise ValueError(f"order must have {B} entries, got {order.numel()}")
         L = _lib.EmdLaunchC(threads or c.threads_per_block, tile_slots, wgs_per_cu,
-                            1 if persistent else 0)
+                            1 if persistent else 0, B)
         _lib.check(_lib.load().emd_run_ragged(
             self._h, _ptr(v), _ptr(off), B, max_n, _ptr(order), ctypes.byref(L), _ptr(imfs),
             _ptr(out_offsets), _ptr(n_imfs), _ptr(overflow), _ptr(dn), _ptr(de), _ptr(dz),

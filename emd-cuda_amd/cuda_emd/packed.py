@@ -209,7 +209,7 @@ class PackedEMD:
             spec = _lib.EmdChunkC(
                 _ptr(self.values).value, _ptr(self.offsets).value, 0, len(idx), bmax,
                 _ptr(self.order_bufs[int(b)]).value,
-                _lib.EmdLaunchC(ch.threads, ch.tile_slots, ch.wgs_per_cu, 1),
+                _lib.EmdLaunchC(ch.threads, ch.tile_slots, ch.wgs_per_cu, 1, B),
                 _ptr(self.imfs).value, None, _ptr(self.n_imfs).value, _ptr(self.overflow).value,
                 _ptr(self.debug_n).value, _ptr(self.debug_extno).value,
                 _ptr(self.debug_nzm).value, _ptr(self.stats).value, None, None)

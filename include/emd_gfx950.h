@@ -46,6 +46,9 @@ typedef struct emd_launch {
   int tile_slots;  /* LDS knot slots (both envelopes); 0 = as emd_config */
   int wgs_per_cu;  /* persistent kernel: resident workgroups per CU (0 = occupancy max) */
   int persistent;  /* 1 = persistent atomic-queue kernel, 0 = one workgroup per trace */
+  int traces;      /* traces the CSR offsets describe (offsets has traces + 1 entries);
+                      0 = the launch's B.  A queue entry outside [0, traces) is skipped:
+                      nothing is read or written for it. */
 } emd_launch;
 
 const char* emd_last_error(void);

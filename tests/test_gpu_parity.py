@@ -27,7 +27,9 @@ def _oracle(signals, max_extrema=None, max_iteration=1000, max_imfs=16):
 
 
 def _assert_parity(signals, gpu, ora, max_imfs=16):
-    bad, worst = compare(list(signals), gpu, ora, max_imfs)
+    import os
+    label = os.environ.get("PYTEST_CURRENT_TEST", "").split(" ")[0]
+    bad, worst = compare(list(signals), gpu, ora, max_imfs, label=label)
     unexplained = [m for m in bad if not m[2]]
     print(f"parity: {len(signals)} traces, {len(bad)} mismatches "
           f"({len(unexplained)} unexplained), worst IMF err {worst:.3e} x max|x|")
@@ -160,5 +162,5 @@ def test_long_trace_parity(threads):
                      for b, s in enumerate(sig)],
                n_imfs=out[2].cpu().numpy(), overflow=out[3].cpu().numpy(),
                debug_n=out[4].cpu().numpy(), debug_extno=out[5].cpu().numpy(),
-               debug_nzm=out[6].cpu().numpy())
+               debug_nzm=out[6].cpu().numpy(), stats=out[7].cpu().numpy())
     _assert_parity(sig, gpu, _oracle(sig, max_extrema=4096))
