@@ -90,7 +90,7 @@ int cap_slots(const emd_config& c, int n) {
 }
 
 struct Shape {
-  int wg, tile, spill_slots, max_n;
+  int wg, tile, spill_slots, spill_rows, max_n;
 };
 
 int check_wg(int wg) {
@@ -125,8 +125,11 @@ int make_shape(const emd_handle* h, int wg, int tile_req, int max_n, Shape* s) {
   // A global region backs sifts whose knots outgrow the LDS tile, and holds the solver
   // rows of systems longer than 64 * (kRegRows + 2) rows (a side has at most
   // min(max_extrema, (max_n + 1) / 2) + 2 * nbsym knots).
+  // A system has m = knots - 2 <= max_rows - 2 rows, and its chunk rows are indexed
+  // below m, so max_rows rows per side and array are enough.
   const int max_rows = std::min(h->cfg.max_extrema, (max_n + 1) / 2) + 2 * h->cfg.nbsym;
   s->spill_slots = (tile < cap || max_rows > 64 * (emdk::kRegRows + 2)) ? cap : 0;
+  s->spill_rows = s->spill_slots ? max_rows : 0;
   s->max_n = std::max(max_n, 1);
   return EMD_OK;
 }
@@ -139,6 +142,7 @@ emdk::SiftParams params(const emd_handle* h, const Shape& s) {
   P.max_imfs = h->cfg.max_imfs;
   P.tile_slots = s.tile;
   P.spill_slots = s.spill_slots;
+  P.spill_rows = s.spill_rows;
   P.max_n = s.max_n;
   P.zero_fill = 1;
   return P;
@@ -266,7 +270,7 @@ struct Plan {
   Shape s;
   emdk::SiftParams P;
   int persistent = 0;
-  int grid = 0;
+  int grid = 0;            // persistent: resident workgroups; plain: traces per sub-launch
   int64_t stride = 0;      // bytes per workgroup global region (0: none needed)
   size_t spill_bytes = 0;  // stride x resident workgroups
 };
@@ -276,7 +280,9 @@ static int make_plan(const emd_handle* h, int B, int max_n, const emd_launch& L,
                       &pl->s);
   if (rc) return rc;
   pl->P = params(h, pl->s);
-  pl->stride = pl->s.spill_slots ? emdk::global_region_bytes(pl->s.spill_slots) : 0;
+  pl->stride = pl->s.spill_slots
+                   ? emdk::global_region_bytes(pl->s.spill_slots, pl->s.spill_rows)
+                   : 0;
   pl->persistent = L.persistent ? 1 : 0;
   if (pl->persistent) {
     const size_t lds = emdk::dyn_lds_bytes(pl->s.tile, pl->P.max_n);
@@ -312,8 +318,8 @@ static int enqueue(emd_handle* h, const Plan& pl, emdk::Batch b, int B, char* sp
     return timed ? mark(h, 1, st) : EMD_OK;
   }
   if (timed && (rc = mark(h, 0, st))) return rc;
-  for (int base = 0; base < B; base += kMaxPlainBlocks) {
-    const int nb = std::min(kMaxPlainBlocks, B - base);
+  for (int base = 0; base < B; base += pl.grid) {
+    const int nb = std::min(pl.grid, B - base);
     b.B = nb;
     b.base = base;
     rc = emdk::launch_sift(0, pl.s.wg, nb, b, pl.P, st);
@@ -454,7 +460,8 @@ int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double
   if (rc) return rc;
   const emdk::SiftParams P = params(h, s);
   const int mbuf = h->cfg.max_extrema + 2 * h->cfg.nbsym;
-  const int64_t stride = s.spill_slots ? emdk::global_region_bytes(s.spill_slots) : 0;
+  const int64_t stride =
+      s.spill_slots ? emdk::global_region_bytes(s.spill_slots, s.spill_rows) : 0;
   if (stride) {
     rc = ensure_spill(h, (size_t)stride * std::min(B, kMaxPlainBlocks));
     if (rc) return rc;

@@ -1598,7 +1598,7 @@ __device__ __forceinline__ WgMem workgroup_mem(double* lds, char* spill, int64_t
     // solver rows after the knot tile (tile_bytes rounding: a multiple of 256 bytes)
     const int64_t tb = ((int64_t)P.spill_slots * kSlotBytes + 255) / 256 * 256;
     m.tg.R = reinterpret_cast<double*>(region + tb);
-    m.tg.rows = P.spill_slots;
+    m.tg.rows = P.spill_rows;
   }
   m.tl.R = m.tg.R;
   m.tl.rows = m.tg.rows;
