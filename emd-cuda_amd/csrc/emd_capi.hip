@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "emd_sift.h"
@@ -59,7 +60,41 @@ struct emd_handle {
   hipEvent_t busy = nullptr;
   hipStream_t busy_stream = nullptr;
   int busy_set = 0;
+  // this device's copy of the static scratch (emd_run_ragged_noalloc)
+  char* static_scratch = nullptr;
 };
+
+// ---- Static scratch of emd_run_ragged_noalloc --------------------------------------------
+// One array per GPU, held until the process ends: the queue counter in its first 256
+// bytes, then the workgroups' global knot tiles and solver rows.  Every call on a GPU
+// shares it (calls that overlap on one GPU race); each call clears the counter first.
+namespace {
+constexpr size_t kStaticScratchBytes = (size_t)EMD_STATIC_SCRATCH_BYTES;
+constexpr size_t kCounterBytes = 256;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+// Device address of the current device's copy of `scratch` (the array's name is only a
+// host-side stand-in: kernels and copies must use this address).  Looked up once per
+// device, with that device current.
+static int static_scratch_address(int device, char** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices)
+    return fail(EMD_ERR_RUNTIME, "device %d outside the %d devices the library tracks", device,
+                kMaxDevices);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(static scratch)");
+    g_scratch_ptr[device] = static_cast<char*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return EMD_OK;
+}
 
 namespace {
 
@@ -235,6 +270,15 @@ int emd_create(const emd_config* cfg, emd_handle** out) {
     (void)hipFree(h->counters);
     delete h;
     return hip_fail(e, "hipEventCreateWithFlags");
+  }
+  {
+    const int rc = static_scratch_address(h->device, &h->static_scratch);
+    if (rc) {
+      (void)hipEventDestroy(h->busy);
+      (void)hipFree(h->counters);
+      delete h;
+      return rc;
+    }
   }
   *out = h;
   return EMD_OK;
@@ -444,6 +488,53 @@ int emd_run_ragged(emd_handle* h, const double* values, const int64_t* offsets, 
   }
   return run_common(h, values, offsets, 0, B, max_n, order, L, imfs, out_offsets, n_imfs,
                     overflow, debug_n, debug_extno, debug_nzm, stats, (hipStream_t)stream);
+}
+
+int emd_run_ragged_noalloc(emd_handle* h, const double* values, const int64_t* offsets, int B,
+                           int max_n, const int* order, const emd_launch* launch, double* imfs,
+                           const int64_t* out_offsets, int* n_imfs, int* overflow, int* debug_n,
+                           int* debug_extno, int* debug_nzm, int* stats, void* stream) {
+  if (!h) return fail(EMD_ERR_VALUE, "null handle");
+  if (!offsets) return fail(EMD_ERR_VALUE, "ragged input needs CSR offsets");
+  if (max_n < 0 || max_n > emdk::kMaxN)
+    return fail(EMD_ERR_VALUE, "max trace length %d outside [0, %d]", max_n, emdk::kMaxN);
+  emd_launch L;
+  if (launch) {
+    L = *launch;
+  } else {
+    L.threads = h->cfg.threads_per_block;
+    L.tile_slots = h->cfg.tile_slots;
+    L.wgs_per_cu = 0;
+    L.persistent = 1;
+    L.traces = 0;
+  }
+  int rc = check_io(B, max_n, values, imfs, n_imfs, overflow, debug_n, debug_extno, debug_nzm);
+  if (rc) return rc;
+  if (L.traces < 0) return fail(EMD_ERR_VALUE, "launch.traces must be >= 0, got %d", L.traces);
+  h->timed = 0;
+  if (B == 0) return EMD_OK;
+  if (!h->static_scratch) return fail(EMD_ERR_RUNTIME, "static scratch address unknown");
+  Plan pl;
+  if ((rc = make_plan(h, B, max_n, L, &pl))) return rc;
+  const size_t room = kStaticScratchBytes - kCounterBytes;
+  if (pl.stride > 0) {
+    if ((size_t)pl.stride > room)
+      return fail(EMD_ERR_VALUE,
+                  "this launch needs %lld bytes of global knot tile and solver rows per "
+                  "workgroup; the static scratch holds %zu (lower max_extrema or max_n, or "
+                  "use emd_run_ragged)",
+                  (long long)pl.stride, room);
+    // as many workgroups as the scratch holds regions for (fewer resident workgroups or
+    // shorter plain sub-launches; the results do not depend on either)
+    pl.grid = (int)std::min<int64_t>(pl.grid, (int64_t)(room / (size_t)pl.stride));
+    pl.spill_bytes = (size_t)pl.stride * (size_t)pl.grid;
+  }
+  const emdk::Batch b = make_batch(values, offsets, 0, order, L.traces > 0 ? L.traces : B,
+                                   imfs, out_offsets, n_imfs, overflow, debug_n, debug_extno,
+                                   debug_nzm, stats);
+  return enqueue(h, pl, b, B, h->static_scratch + kCounterBytes,
+                 reinterpret_cast<unsigned*>(h->static_scratch), h->timing != 0,
+                 (hipStream_t)stream);
 }
 
 int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double* max_env,

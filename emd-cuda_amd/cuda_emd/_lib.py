@@ -57,6 +57,9 @@ PROTOTYPES = {
     "emd_run_ragged": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp,
                                       ctypes.POINTER(EmdLaunchC), _vp, _vp, _vp, _vp, _vp, _vp,
                                       _vp, _vp, _vp]),
+    "emd_run_ragged_noalloc": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp,
+                                              ctypes.POINTER(EmdLaunchC), _vp, _vp, _vp, _vp,
+                                              _vp, _vp, _vp, _vp, _vp]),
     "emd_envelope_only": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
                                          _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "emd_max_resident": (ctypes.c_int, [_vp, ctypes.POINTER(EmdLaunchC), ctypes.c_int,

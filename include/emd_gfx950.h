@@ -93,6 +93,24 @@ int emd_run_ragged(emd_handle* h, const double* values, const int64_t* offsets, 
                    const int64_t* out_offsets, int* n_imfs, int* overflow, int* debug_n,
                    int* debug_extno, int* debug_nzm, int* stats, void* stream);
 
+/* emd_run_ragged without any device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, outputs and queue semantics
+ * as emd_run_ragged; the caller passes every input and output.  Instead of the handle's
+ * scratch (grown by hipMalloc on demand), the queue counter and the workgroups' global
+ * knot tiles and solver rows live in one static device array of EMD_STATIC_SCRATCH_BYTES
+ * per GPU, held until the process ends and looked up by emd_create.  A launch whose
+ * workgroups would need more than the array holds runs with fewer resident workgroups
+ * (or shorter plain sub-launches), which never changes results; one whose single
+ * workgroup region exceeds it is rejected with EMD_ERR_VALUE.  Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU (another stream, another handle, another
+ * thread) race, so the caller orders them.  Nothing is allocated or freed. */
+#define EMD_STATIC_SCRATCH_BYTES (128ll << 20)
+int emd_run_ragged_noalloc(emd_handle* h, const double* values, const int64_t* offsets, int B,
+                           int max_n, const int* order, const emd_launch* launch, double* imfs,
+                           const int64_t* out_offsets, int* n_imfs, int* overflow, int* debug_n,
+                           int* debug_extno, int* debug_nzm, int* stats, void* stream);
+
 /* Replaces EMDBatchLauncher.run_envelope_only -> envelope_batch_kernel
  * (kernels.py:1374-1407, 1100-1166): extrema + mirror + both splines, no sifting.
  * max_env/min_env [B,N]; n_extrema [B,2]; degenerate [B]; knot positions/values
