@@ -70,7 +70,15 @@ def _cpu_one(x):
     return O.emd(x, max_imf=15).n_imfs
 
 
+def _cpu_record(x):
+    from oracle import pyemd_restated as O
+    return O.emd(x, max_imf=15)
+
+
 def cpu_baseline(x: np.ndarray, n: int):
+    """The oracle timed on the first n traces with a process pool (the reported CPU
+    baseline), then run again untimed to return its full records for the parity check of
+    the GPU output on the same traces."""
     import multiprocessing as mp
     cores = min(16, os.cpu_count() or 1)
     sample = x[:n]
@@ -80,9 +88,39 @@ def cpu_baseline(x: np.ndarray, n: int):
         t0 = time.perf_counter()
         pool.map(_cpu_one, sample, chunksize=max(1, n // (4 * cores)))
         dt = time.perf_counter() - t0
+        records = pool.map(_cpu_record, sample, chunksize=max(1, n // (4 * cores)))
     return {"value": n / dt, "unit": "traces/s", "cores": cores, "kind": "port",
             "sample": f"first {n} of the {len(x)} seismic traces (N={x.shape[1]}), "
-                      f"NumPy/SciPy PyEMD EMD() restatement, multiprocessing pool of {cores}"}
+                      f"NumPy/SciPy PyEMD EMD() restatement, multiprocessing pool of {cores}"}, records
+
+
+def ragged_records(out, lengths, count=None):
+    """run_ragged's tuple as the per-trace dict tests/parity.py compares (first count
+    traces)."""
+    count = len(lengths) if count is None else count
+    imfs, oo = out[0].cpu().numpy(), out[1].cpu().numpy()
+    n = out[2].cpu().numpy()
+    rows = [imfs[oo[s]:oo[s] + n[s] * lengths[s]].reshape(n[s], lengths[s])
+            for s in range(count)]
+    return dict(imfs=rows, n_imfs=n[:count], overflow=out[3].cpu().numpy()[:count],
+                debug_n=out[4].cpu().numpy()[:count], debug_extno=out[5].cpu().numpy()[:count],
+                debug_nzm=out[6].cpu().numpy()[:count], stats=out[7].cpu().numpy()[:count])
+
+
+def parity_vs_oracle(x, out, lengths, records):
+    """GPU output of the timed path against the oracle on the CPU-baseline traces."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from parity import compare
+    n = len(records)
+    bad, worst = compare(list(x[:n]), ragged_records(out, lengths, n), records,
+                         label="bench:seismic")
+    traces_bad = sorted({m[0] for m in bad})
+    unexplained = sorted({m[0] for m in bad if not m[2]})
+    return {"traces": n, "mismatched_traces": len(traces_bad),
+            "tie_explained": len(traces_bad) - len(unexplained),
+            "unexplained": len(unexplained), "worst_imf_err_rel": worst,
+            "bar": "n_imfs, overflow, debug_n/extno/nzm and sift/peak/knot counters "
+                   "identical; IMF max-abs <= 1e-9 max|x|"}
 
 
 def train_cost_model(launcher, args):
@@ -194,13 +232,7 @@ def _progress(msg):
 def _seismic_chunks(total, seed, chunk=16384):
     """total seismic traces (N=2001) drawn chunk by chunk from one generator."""
     from cuda_emd import synth
-    rng = np.random.default_rng(seed)
-    parts, left = [], total
-    while left > 0:
-        parts.append(synth.seismic_traces(min(chunk, left), 2001, rng=rng))
-        left -= len(parts[-1])
-        _progress(f"seismic seed {seed}: {total - left}/{total}")
-    return np.concatenate(parts)
+    return synth.seismic_chunks(total, seed, chunk=chunk, progress=_progress)
 
 
 def bench_costmodel(args):
@@ -232,17 +264,10 @@ def bench_costmodel(args):
     fit_s = time.perf_counter() - t1
     _progress(f"fit done in {fit_s:.1f}s")
     # held-out: 95% seismic (N=2001) + 5% white N(0,1) at N=8192, shuffled
-    rng = np.random.default_rng(4)
-    n_white = int(round(0.05 * args.eval_total))
-    xs = _seismic_chunks(args.eval_total - n_white, 4)
-    traces = [xs[k] for k in range(len(xs))] + [rng.standard_normal(8192) for _ in range(n_white)]
-    kinds = np.array([0] * len(xs) + [1] * n_white)
-    perm = rng.permutation(len(traces))
-    traces = [traces[k] for k in perm]
-    kinds = kinds[perm]
-    offsets = np.zeros(len(traces) + 1, np.int64)
-    offsets[1:] = np.cumsum([len(t) for t in traces])
-    values = np.concatenate(traces)
+    from cuda_emd import synth
+    values, offsets, kinds = synth.heldout_mix(args.eval_total, seed=4, progress=_progress)
+    n_white = int((kinds == 1).sum())
+    n_seis = len(kinds) - n_white
     dv, doff = torch.from_numpy(values).cuda(), torch.from_numpy(offsets).cuda()
     tuner = Autotuner(cfg, cache_path=args.tune_cache)
     res = {}
@@ -273,14 +298,14 @@ def bench_costmodel(args):
     pred = cm.predict(costmodel.features(dv, doff))
     same = all(np.array_equal(a, b) for a, b in zip(res["fifo"][1][:5], out[:5])) and \
         all(np.array_equal(a, b) for a, b in zip(res["plain"][1][:5], out[:5]))
-    B = len(traces)
+    B = len(kinds)
     line = {
         "metric": "cost-model MAPE on held-out traces (sifts, n_imfs, peak extrema)",
         "value": costmodel.mape(pred, true)["sifts"], "unit": "MAPE (sifts)",
         "higher_is_better": False, "n_gpus": 1, "dtype": "f32 MLP / f64 EMD",
         "data": "synthetic",
         "config": {"workload": f"train {args.train_total} seismic (seed 3); eval {B} = "
-                               f"{len(xs)} seismic (seed 4) + {n_white} white N=8192"},
+                               f"{n_seis} seismic (seed 4) + {n_white} white N=8192"},
         "mape_all": costmodel.mape(pred, true),
         "mape_seismic": costmodel.mape(pred[kinds == 0], true[kinds == 0]),
         "mape_white": costmodel.mape(pred[kinds == 1], true[kinds == 1]),
@@ -486,7 +511,7 @@ def main():
     x = synth.seismic_traces(args.batch, args.n, seed=1)
     B, N = x.shape
     # CPU baseline first: its worker processes fork before this process touches the GPU
-    cpu = None if args.no_cpu else cpu_baseline(x, args.cpu_sample)
+    cpu, cpu_records = (None, None) if args.no_cpu else cpu_baseline(x, args.cpu_sample)
 
     import torch
     from cuda_emd import EMDKernelConfig
@@ -594,6 +619,29 @@ def main():
         model_info["mape_on_workload"] = costmodel.mape(
             cm.predict(costmodel.features(dx.view(-1), doff)), true)
 
+    if args.mode == "both":
+        from cuda_emd.packed import Autotuner, PipelinedEMD
+        # Same-shape comparators.  plain_same_shape: the packed path's launch shape (its
+        # workgroup size and LDS tile, so the same workgroups per CU) with one workgroup
+        # per trace instead of the persistent queue.  packed_no_model: the same pipelined
+        # persistent path in batch order with the tile the autotuner's "largest tile for 4
+        # workgroups per CU" rule gives, i.e. everything but the cost model.
+        tile_used = int(plan_tile[-1]) if plan_tile else 0
+        out, ms_wall, ms_ev = timed(lambda: sift_call(lambda: launcher.run_ragged(
+            dx.view(-1), offsets, threads=args.threads, tile_slots=tile_used,
+            persistent=False)))
+        results["plain_same_shape"] = dict(ms=ms_wall, kernel_ms=ms_ev,
+                                           n_imfs=out[2].cpu().numpy(),
+                                           stats=out[7].cpu().numpy(), out=out, tile=tile_used)
+        tile4 = Autotuner._largest_tile_for(launcher, args.threads, N, 4)
+        pipe0 = PipelinedEMD(cfg, None, threads=args.threads, tile_slots=tile4,
+                             depth=args.lanes, timing=True)
+        out, ms_wall, ms_ev = timed(lambda: pipe0.submit(dx.view(-1), offsets, offsets_dev=doff,
+                                                         lengths_dev=dlen), pipe0)
+        results["packed_no_model"] = dict(ms=ms_wall, kernel_ms=ms_ev,
+                                          n_imfs=out[2].cpu().numpy(),
+                                          stats=out[7].cpu().numpy(), out=out, tile=tile4)
+
     head = next(k for k in ("packed", "fifo", "plain") if k in results)
     r = results[head]
     flops = algorithmic_flops(r["stats"], r["n_imfs"], lengths)
@@ -661,6 +709,24 @@ def main():
             and np.array_equal(results["plain"]["stats"][:, :3], r["stats"][:, :3]))
     if "packed" in results and "fifo" in results:
         line["speedup_vs_fifo"] = results["fifo"]["ms"] / results["packed"]["ms"]
+    for name in ("plain_same_shape", "packed_no_model"):
+        if name in results and "packed" in results:
+            q = results[name]
+            line[name] = {"traces_per_s": B / (q["ms"] * 1e-3), "ms_per_step": q["ms"],
+                          "kernel_ms": q["kernel_ms"], "tile_slots": q["tile"],
+                          "threads": args.threads,
+                          "identical": bool(all(np.array_equal(
+                              q["out"][k].cpu().numpy(), results["packed"]["out"][k].cpu().numpy())
+                              for k in range(2, 7)) and np.array_equal(
+                              q["stats"][:, :3], results["packed"]["stats"][:, :3]))}
+    if "plain_same_shape" in results and "packed" in results:
+        line["speedup_vs_plain_same_shape"] = (results["plain_same_shape"]["ms"]
+                                               / results["packed"]["ms"])
+    if "packed_no_model" in results and "packed" in results:
+        line["speedup_vs_no_model"] = results["packed_no_model"]["ms"] / results["packed"]["ms"]
+    if cpu_records is not None and "packed" in results:
+        line["parity_vs_oracle"] = parity_vs_oracle(x, results["packed"]["out"], lengths,
+                                                    cpu_records)
     if model_info is not None:
         line["cost_model"] = model_info
     line["sifts_per_trace"] = float(r["stats"][:, 0].mean())

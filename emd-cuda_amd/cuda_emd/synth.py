@@ -77,6 +77,39 @@ def seismic_traces(B: int = 16384, N: int = 2001, seed: int = 1, rng=None) -> np
     return np.ascontiguousarray(clean + scale * noise)
 
 
+def seismic_chunks(total: int, seed: int, N: int = 2001, chunk: int = 16384,
+                   progress=None) -> np.ndarray:
+    """``total`` seismic traces (as seismic_traces) drawn chunk by chunk of ``chunk``
+    traces from one generator seeded ``seed`` (the cost-model workload's training and
+    held-out seismic traces)."""
+    rng = np.random.default_rng(seed)
+    parts, left = [], total
+    while left > 0:
+        parts.append(seismic_traces(min(chunk, left), N, rng=rng))
+        left -= len(parts[-1])
+        if progress is not None:
+            progress(f"seismic seed {seed}: {total - left}/{total}")
+    return np.concatenate(parts) if parts else np.zeros((0, N))
+
+
+def heldout_mix(total: int = 262144, seed: int = 4, white_frac: float = 0.05,
+                white_N: int = 8192, progress=None):
+    """Held-out set of the cost-model workload (262144 traces, seed 4): seismic traces
+    (N=2001, seismic_chunks with ``seed``) plus ``white_frac`` N(0,1) white noise of length
+    ``white_N``, shuffled.  A second generator seeded ``seed`` draws the white traces and
+    then the permutation.  Returns (values, offsets, kinds) with kinds 0=seismic 1=white."""
+    rng = np.random.default_rng(seed)
+    n_white = int(round(white_frac * total))
+    xs = seismic_chunks(total - n_white, seed, progress=progress)
+    traces = [xs[k] for k in range(len(xs))] + [rng.standard_normal(white_N)
+                                                for _ in range(n_white)]
+    kinds = np.array([0] * len(xs) + [1] * n_white, np.int8)
+    perm = rng.permutation(len(traces))
+    traces = [traces[k] for k in perm]
+    values, offsets = _csr(traces)
+    return values, offsets, kinds[perm]
+
+
 def ar1(N: int, phi: float, rng) -> np.ndarray:
     """Unit-variance AR(1): x[0] ~ N(0,1), x[n] = phi x[n-1] + sqrt(1-phi^2) e[n]."""
     e = rng.standard_normal(N)

@@ -20,6 +20,10 @@ import os
 import numpy as np
 
 IMF_TOL = 1e-9
+# worst IMF error (x max|x|) of the last compare(), over all traces and over the traces
+# whose decomposition did not overflow (an overflowed decomposition keeps clamped knot
+# lists, whose far-reaching mirrored knots can make IMFs hundreds of times max|x|)
+LAST_WORST = {"all": 0.0, "no_overflow": 0.0}
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LOG_PATH = os.environ.get("EMD_PARITY_LOG",
                           os.path.join(ROOT, "gpurun_out", "parity_divergences.jsonl"))
@@ -67,7 +71,7 @@ def compare(signals, gpu, ora, max_imfs=16, label=""):
     [B,4] optional); ora: oracle results (EMDResult list).  Returns (mismatches,
     worst_rel_err); a mismatch is (index, field, explained_by_near_tie)."""
     bad = []
-    worst = 0.0
+    worst = worst_ok = 0.0
     stats = gpu.get("stats")
     for b, x in enumerate(signals):
         r = ora[b]
@@ -96,6 +100,8 @@ def compare(signals, gpu, ora, max_imfs=16, label=""):
             scale = scale if scale > 0 else 1.0
             err = float(np.max(np.abs(g - r.imfs))) / scale if k else 0.0
             worst = max(worst, err)
+            if not r.overflow:
+                worst_ok = max(worst_ok, err)
             if not (err <= IMF_TOL):
                 fields.append(f"imfs(err={err:.3e})")
                 for f in fields:
@@ -111,4 +117,5 @@ def compare(signals, gpu, ora, max_imfs=16, label=""):
               "gpu": {"n_imfs": int(gpu["n_imfs"][b]), "debug_n": g_rec[0].tolist()},
               "oracle": {"n_imfs": r.n_imfs, "debug_n": o_rec[0].tolist()},
               "ties": [[t.imf, t.sift, t.what, t.value, t.threshold] for t in ties[:8]]})
+    LAST_WORST["all"], LAST_WORST["no_overflow"] = worst, worst_ok
     return bad, worst

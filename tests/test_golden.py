@@ -82,3 +82,27 @@ def test_gpu_matches_golden():
         n = int(g["n_imfs"][k])
         got = imfs[oo[k]:oo[k] + n * L].reshape(n, L)
         assert np.max(np.abs(got - _rows(g, k, ro)), initial=0.0) <= 1e-9 * scale, name
+
+
+CONFIGS = os.path.join(os.path.dirname(__file__), "golden", "configs_sample.npz")
+TIE_KINDS = ("sample_sign", "difference_sign", "total_power_1e-10", "svar", "std",
+             "energy_ratio", "range", "total_power")
+
+
+def test_oracle_reproduces_config_sample():
+    """The oracle still gives the records frozen for the ragged-mix, held-out and stress
+    samples (tests/golden/make_configs.py), near ties included."""
+    g = np.load(CONFIGS, allow_pickle=False)
+    off = g["offsets"]
+    ties = g["ties"]
+    for s in range(len(g["n_imfs"])):
+        r = O.emd(g["values"][off[s]:off[s + 1]], max_imf=15,
+                  max_extrema=int(g["max_extrema"][s]), debug_slots=16)
+        assert r.n_imfs == g["n_imfs"][s] and r.overflow == g["overflow"][s], s
+        assert np.array_equal(r.debug_n, g["debug_n"][s]), s
+        assert np.array_equal(r.debug_extno, g["debug_extno"][s]), s
+        assert np.array_equal(r.debug_nzm, g["debug_nzm"][s]), s
+        assert [r.total_sifts, r.peak_extrema, r.knot_sum] == g["counters"][s].tolist(), s
+        mine = sorted((t.imf, t.sift, TIE_KINDS.index(t.what)) for t in r.near_ties)
+        frozen = sorted(map(tuple, ties[ties[:, 0] == s][:, 1:].tolist()))
+        assert mine == frozen, s
