@@ -81,6 +81,7 @@ struct SiftShared {
   int tot_max, tot_min;
   // mirror results
   int k0[2], nk[2], kind[2], mtot;
+  int xf[2], xl[2];  // first and last true knot position per side
   int bad;
   double p3[2][4];
   double carry[2][3];  // sum of squares, max, min of the IMF left by sift n (slot n & 1)
@@ -130,13 +131,11 @@ __device__ __forceinline__ int fresh_tid() {
   return t;
 }
 
-// 1/q by the hardware reciprocal and two Newton steps (within an ulp or two; only the
-// std-test sum uses it)
+// 1/q by the hardware reciprocal and one Newton step (the hardware value is good to about
+// 2^-50 relative, so one step lands within an ulp or two; only the std-test sum uses it)
 __device__ __forceinline__ double recip(double q) {
-  double y = __builtin_amdgcn_rcp(q);
-  double e = __builtin_fma(-q, y, 1.0);
-  y = __builtin_fma(y, e, y);
-  e = __builtin_fma(-q, y, 1.0);
+  const double y = __builtin_amdgcn_rcp(q);
+  const double e = __builtin_fma(-q, y, 1.0);
   return __builtin_fma(y, e, y);
 }
 
@@ -163,12 +162,15 @@ constexpr int kDppRowShr = 0x110;     // + n: row_shr:n (lane i <- lane i-n of i
 constexpr int kDppWaveShl1 = 0x130;   // wave_shl:1 (lane i <- lane i+1)
 constexpr int kDppWaveShr1 = 0x138;   // wave_shr:1 (lane i <- lane i-1)
 
-// Lanes of rows outside ROWS keep 0 (the `old` operand).
+// A lane whose source lane does not exist (wave shifts past the ends) reads 0; lanes of
+// rows outside ROWS are left undefined, which only the row broadcasts of
+// wave_tree_sum_last use (their result is read from lane 63 alone).  With no `old`
+// operand the compiler emits no register initialisation per move.
 template <int CTRL, int ROWS = 0xF>
 __device__ __forceinline__ double dpp_d(double v) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROWS, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWS, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, ROWS, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, ROWS, 0xF, true);
   return __longlong_as_double(
       (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo));
 }
@@ -208,8 +210,29 @@ __device__ __forceinline__ double wave_tree_sum_last(double v) {
   v += dpp_d<kDppBcast31, 0x8>(v);
   return v;
 }
+// Three wave_tree_sum_last trees, level by level (independent moves fill each other's
+// DPP read hazards).
+__device__ __forceinline__ void wave_tree_sum3_last(double& a, double& b, double& c) {
+#define EMD_LEVEL(CTRL, ROWS)        \
+  {                                  \
+    const double ta = dpp_d<CTRL, ROWS>(a); \
+    const double tb = dpp_d<CTRL, ROWS>(b); \
+    const double tc = dpp_d<CTRL, ROWS>(c); \
+    a += ta;                         \
+    b += tb;                         \
+    c += tc;                         \
+  }
+  EMD_LEVEL(kDppXor1, 0xF)
+  EMD_LEVEL(kDppXor2, 0xF)
+  EMD_LEVEL(kDppHalfMirror, 0xF)
+  EMD_LEVEL(kDppMirror, 0xF)
+  EMD_LEVEL(kDppBcast15, 0xA)
+  EMD_LEVEL(kDppBcast31, 0x8)
+#undef EMD_LEVEL
+}
 // Inclusive prefix sum over the 64 lanes: row-local shifts by 1, 2, 4 and 8 (a lane whose
-// source lies outside its 16-lane row adds 0), then the row totals by broadcast.
+// source lies outside its 16-lane row adds 0), then the row totals by broadcast (rows
+// outside ROWS add 0: the `old` operand).
 template <int CTRL, int ROWS = 0xF>
 __device__ __forceinline__ unsigned dpp_u(unsigned v) {
   return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, false);
@@ -255,6 +278,32 @@ struct DetectResult {
 
 __device__ __forceinline__ uint64_t lanemask_lt(int lane) { return (uint64_t(1) << lane) - 1; }
 
+// Lane masks straight from the compare (one v_cmp into a scalar register pair; the masks
+// are then combined on the scalar unit).  Predicates: LLVM's fcmp / icmp codes.
+constexpr int kFOeq = 1, kFOgt = 2, kFOlt = 4, kFUne = 14, kISlt = 40;
+template <int PRED>
+__device__ __forceinline__ uint64_t fmask(double a) {
+  return __builtin_amdgcn_fcmp(a, 0.0, PRED);
+}
+__device__ __forceinline__ uint64_t lt_mask(int a, int b) {
+  return __builtin_amdgcn_sicmp(a, b, kISlt);
+}
+// Lanes 0..n-1 of a 64-lane block (n <= 0: none, n >= 64: all), on the scalar unit.
+__device__ __forceinline__ uint64_t lanes_below(int n) {
+  return n <= 0 ? uint64_t(0) : n >= 64 ? ~uint64_t(0) : (uint64_t(1) << n) - 1;
+}
+// A 64-bit value that is equal in every lane, as a scalar.
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+  return (uint64_t(hi) << 32) | lo;
+}
+// Set bits of a scalar lane mask below this lane.
+__device__ __forceinline__ int mbcnt(uint64_t m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
 // Serial scan on one lane over a staged copy of the IMF: emit(idx) for every maximum
 // (want_max) or every minimum, in index order.
 template <class Emit>
@@ -295,30 +344,31 @@ __device__ __forceinline__ void serial_extrema(const double* S, int N, bool want
 
 template <int WG>
 __device__ __forceinline__ void compact_knots(const Work& wk, int N, const Tile& t, int nm,
-                                              int nn, int pad, int& bad) {
+                                              int nn, int pad, uint64_t& bad) {
   constexpr int NW = WG / 64;
-  const int tid = fresh_tid(), lane = tid & 63, w = tid >> 6;
+  const int tid = fresh_tid(), lane = tid & 63, w = uni(tid >> 6);
   const int nblk = ceil_div(N, 64);
   const int tb0 = pad, tb1 = 3 * pad + nm;
-  const uint64_t lt = lanemask_lt(lane);
   for (int blk = w; blk < nblk; blk += NW) {
     const int i = blk * 64 + lane;
-    // every load of the block is issued before the first use (one LDS round trip)
+    // every load of the block is issued before the first use (one LDS round trip); the
+    // masks stay in vector registers, so no scalar read waits for them
     const uint64_t bm = wk.mmax[blk], bn = wk.mmin[blk];
     const int pm = wk.pmax[blk], pn = wk.pmin[blk];
     const double y = wk.imf[i];
-    const int km = pm + __popcll(bm & lt);
-    const int kn = pn + __popcll(bn & lt);
-    if (((bm >> lane) & 1ull) && km < nm) {
+    const int km = pm + mbcnt(bm);
+    const int kn = pn + mbcnt(bn);
+    const bool sm = ((bm >> lane) & 1ull) && km < nm;  // this lane stores a maximum
+    const bool sn = ((bn >> lane) & 1ull) && kn < nn;  // or a minimum (max_extrema clamp)
+    if (sm) {
       t.X[tb0 + km] = i;
       t.Y[tb0 + km] = y;
-      bad |= (y < 0.0);
     }
-    if (((bn >> lane) & 1ull) && kn < nn) {
+    if (sn) {
       t.X[tb1 + kn] = i;
       t.Y[tb1 + kn] = y;
-      bad |= (y > 0.0);
     }
+    bad |= (__ballot(sm) & fmask<kFOlt>(y)) | (__ballot(sn) & fmask<kFOgt>(y));
   }
 }
 
@@ -327,36 +377,37 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
                                                const Tile& tl, const Tile& tg,
                                                const SiftParams& P) {
   constexpr int NW = WG / 64;
-  const int tid = fresh_tid(), lane = tid & 63, w = tid >> 6;
+  const int tid = fresh_tid(), lane = tid & 63, w = uni(tid >> 6);
   const int pad = P.nbsym;
   const int nblk = ceil_div(N, 64);
   const double* imf = wk.imf;
 
-  // ---- pass A: per-block ballots of maxima/minima, zero crossings, plateaus ----------
+  // ---- pass A: per-block lane masks of maxima/minima, zero crossings, plateaus --------
   int nz = 0, plat = 0;
   for (int blk = w; blk < nblk; blk += NW) {
     const int i = blk * 64 + lane;
     // all three loads stay inside the work area (i + 1 <= 64 * nbcap lands in the block
-    // sums that follow the IMF); values outside the trace are masked by the predicates
+    // sums that follow the IMF); values outside the trace are masked off below
     const double x = imf[i];
     const double prev = imf[i >= 1 ? i - 1 : 0];
     const double next = imf[i + 1];
-    const bool in = i < N, inner = i <= N - 2;
     const double d2 = next - x;
     const double d1 = x - prev;
-    const double p = d1 * d2;
-    const bool ext = in && inner && i >= 1 && (p < 0.0);
-    const bool is_max = ext && (d1 > 0.0);
-    const bool is_min = ext && (d1 < 0.0);
-    const bool pl = in && inner && (d2 == 0.0);
-    const bool zc = in && inner && (x * next < 0.0);
-    const bool zr = in && (x == 0.0) && (i == 0 || prev != 0.0);
-    const uint64_t bm = __ballot(is_max), bn = __ballot(is_min);
-    nz += __popcll(__ballot(zc)) + __popcll(__ballot(zr));
-    plat |= (__ballot(pl) != 0ull);
+    // lanes whose sample is in the trace (i < N), has a successor (i <= N - 2) and, for
+    // extrema, a predecessor (i >= 1)
+    const uint64_t in = lanes_below(N - blk * 64), inner = lanes_below(N - 1 - blk * 64);
+    const uint64_t inner1 = blk == 0 ? inner & ~uint64_t(1) : inner;
+    const uint64_t ext = fmask<kFOlt>(d1 * d2) & inner1;
+    const uint64_t bm = ext & fmask<kFOgt>(d1), bn = ext & fmask<kFOlt>(d1);
+    const uint64_t zc = fmask<kFOlt>(x * next) & inner;
+    const uint64_t zr = fmask<kFOeq>(x) & in & (fmask<kFUne>(prev) | (blk == 0 ? 1ull : 0ull));
+    nz += __popcll(zc) + __popcll(zr);
+    plat |= (fmask<kFOeq>(d2) & inner) != 0ull;
     if (lane == 0) {
       wk.mmax[blk] = bm;
       wk.mmin[blk] = bn;
+      // the block's counts (maxima | minima << 16) for the scan below
+      wk.pmax[blk] = __popcll(bm) | (__popcll(bn) << 16);
     }
   }
   if (lane == 0) {
@@ -388,19 +439,27 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
     // priority lets it go ahead of the other workgroups' waves sharing its SIMD.
     __builtin_amdgcn_s_setprio(2);
     const int per = ceil_div(nblk, 64);
+    // block counts: written by pass A, or recounted from the masks the plateau rules
+    // rewrote
+    const bool recount = uni(plt) != 0;
+    auto count = [&](int b) -> unsigned {
+      return recount ? (__popcll(wk.mmax[b]) | (unsigned(__popcll(wk.mmin[b])) << 16))
+                     : unsigned(wk.pmax[b]);
+    };
     unsigned loc = 0;
     for (int k = 0; k < per; ++k) {
       const int b = lane * per + k;
-      if (b < nblk) loc += __popcll(wk.mmax[b]) | (unsigned(__popcll(wk.mmin[b])) << 16);
+      if (b < nblk) loc += count(b);
     }
     const unsigned incl = wave_incl_scan(loc);
     unsigned run = incl - loc;
     for (int k = 0; k < per; ++k) {
       const int b = lane * per + k;
       if (b < nblk) {
+        const unsigned c = count(b);
         wk.pmax[b] = int(run & 0xffffu);
         wk.pmin[b] = int(run >> 16);
-        run += __popcll(wk.mmax[b]) | (unsigned(__popcll(wk.mmin[b])) << 16);
+        run += c;
       }
     }
     if (lane == 63) {
@@ -420,10 +479,10 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
   R.in_lds = (4 * pad + R.nm + R.nn <= P.tile_slots);
 
   // ---- pass B: compact the extrema into the knot tile ---------------------------------
-  int bad = 0;
+  uint64_t bad = 0;
   if (R.in_lds) compact_knots<WG>(wk, N, tl, R.nm, R.nn, pad, bad);
   else compact_knots<WG>(wk, N, tg, R.nm, R.nn, pad, bad);
-  const int bw = (__ballot(bad) != 0ull) ? 1 : 0;
+  const int bw = bad != 0ull ? 1 : 0;
   if (lane == 0) sh.bad_w[w] = bw;
   __syncthreads();
   if (tid == 0) {
@@ -616,11 +675,15 @@ __device__ __forceinline__ void finish_lists(const Tile& t, const KnotEnds& c, i
 // the reference's mirror_one_side (kernels.py:360-558) follows the same branch table.
 __device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int nb, double S0,
                                        double SN, int* k0, int* nk, int* kind, int& bad,
-                                       double (*p3)[4]) {
+                                       double (*p3)[4], int* xf, int* xl) {
   const int tb0 = nb, tb1 = 3 * nb + nm;
   const KnotEnds c = load_ends(t, tb0, tb1, nm, nn, nb + 1, fresh_tid() & 63, N, S0, SN);
   const int Xm0 = ends_x(c, 0, 0), XmL = ends_x(c, 0, nm - 1);
   const int Xn0 = ends_x(c, 1, 0), XnL = ends_x(c, 1, nn - 1);
+  xf[0] = Xm0;
+  xl[0] = XmL;
+  xf[1] = Xn0;
+  xl[1] = XnL;
   const double Ym0 = ends_y(c, 0, 0), YmL = ends_y(c, 0, nm - 1);
   const double Yn0 = ends_y(c, 1, 0), YnL = ends_y(c, 1, nn - 1);
   KDesc lmax, lmin, rmax, rmin;
@@ -709,7 +772,7 @@ __device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int
 // Per interval k (knots k, k+1) of a not-a-knot side: H[k] = 1/h and, until the solve
 // overwrites it, M[k] = the chord slope (y_{k+1} - y_k) / h as (y_{k+1} - y_k) * (1/h).
 __device__ __forceinline__ void interval_setup(const Tile& t, int k) {
-  const double ih = 1.0 / double(t.X[k + 1] - t.X[k]);
+  const double ih = recip(double(t.X[k + 1] - t.X[k]));
   t.H[k] = ih;
   t.M[k] = (t.Y[k + 1] - t.Y[k]) * ih;
 }
@@ -748,7 +811,7 @@ __device__ __forceinline__ void sweep_row(const Tile& t, int k0, int m, int j, b
                                           double& w) {
   double a, b, c, d;
   nak_row(t, k0, m, j, a, b, c, d);
-  const double rb = 1.0 / (first ? b : b - a * g);
+  const double rb = recip(first ? b : b - a * g);
   const double gn = last ? 0.0 : c * rb;
   const double un = (first ? d : d - a * u) * rb;
   const double vn = (first ? a : -(a * v)) * rb;
@@ -853,8 +916,8 @@ __device__ __forceinline__ void spline_system_wave(const Tile& t, int side, int 
   const double nsa = dpp_d<kDppWaveShl1>(rsa), nsb = dpp_d<kDppWaveShl1>(rsb);
   const double nsc = dpp_d<kDppWaveShl1>(rsc), nsd = dpp_d<kDppWaveShl1>(rsd);
   const bool has_next = lane + 1 < K;
-  const double k1 = rea / rsb;
-  const double k2 = has_next ? rec / nsb : 0.0;
+  const double k1 = rea * recip(rsb);
+  const double k2 = has_next ? rec * recip(nsb) : 0.0;
   const double A = -(k1 * rsa);
   const double B = reb - k1 * rsc - (has_next ? k2 * nsa : 0.0);
   const double C = has_next ? -(k2 * nsc) : 0.0;
@@ -862,7 +925,7 @@ __device__ __forceinline__ void spline_system_wave(const Tile& t, int side, int 
   // parallel cyclic reduction over the chunk e-rows, each row scaled to a unit diagonal
   // (alpha x_{l-dd} + x_l + gamma x_{l+dd} = delta), so a level exchanges three values
   // each way and divides once; distance 1 by whole-wave DPP shifts
-  double ib = 1.0 / B;
+  double ib = recip(B);
   double al = A * ib, ga = C * ib, de = D * ib;
   for (int dd = 1; dd < K; dd <<= 1) {
     double au, gu, du, ad, gd, ddn;
@@ -889,7 +952,7 @@ __device__ __forceinline__ void spline_system_wave(const Tile& t, int side, int 
     const double nC = -(ga * gd0);
     const double nB = __builtin_fma(-ga, ad0, __builtin_fma(-al, gu0, 1.0));
     const double nD = __builtin_fma(-ga, dd0, __builtin_fma(-al, du0, de));
-    ib = 1.0 / nB;
+    ib = recip(nB);
     al = nA * ib;
     ga = nC * ib;
     de = nD * ib;
@@ -897,7 +960,7 @@ __device__ __forceinline__ void spline_system_wave(const Tile& t, int side, int 
   const double xe = de;
   const double xprev = dpp_d<kDppWaveShr1>(xe);
   if (act) {
-    const double xs = (rsd - (lane > 0 ? rsa * xprev : 0.0) - rsc * xe) / rsb;
+    const double xs = (rsd - (lane > 0 ? rsa * xprev : 0.0) - rsc * xe) * recip(rsb);
     t.M[k0 + 1 + s] = xs;
     t.M[k0 + 1 + e] = xe;
     if (in_regs) {
@@ -940,7 +1003,7 @@ __device__ __forceinline__ void solve_side(const Tile& t, int side, int k0, int 
 template <int WG>
 __device__ __forceinline__ void spline_solve(const Tile& t, const SiftShared<WG>& sh,
                                              const DetectResult& D, int nb) {
-  const int tid = fresh_tid(), lane = tid & 63, wv = tid >> 6;
+  const int tid = fresh_tid(), lane = tid & 63, wv = uni(tid >> 6);
   const int mtot = uni(sh.mtot);
   if (mtot > 0) {
     // wave 0 solves the maxima envelope and wave 1 the minima (wave 0 both when WG = 64);
@@ -1038,41 +1101,91 @@ __device__ __forceinline__ double blocks_tree(const double* v, int nblk) {
   return lane_d(wave_tree_sum_last(a), 63);
 }
 
+// blocks_tree of three rows of block partials (row stride `stride`) at once: the same
+// leaves, pairs and order as three blocks_tree calls, with the three DPP trees interleaved.
+__device__ __forceinline__ void blocks_tree3(const double* v, int stride, int nblk, double& s0,
+                                             double& s1, double& s2) {
+  const int lane = fresh_tid() & 63;
+  int P = 1;
+  while (P < nblk) P <<= 1;
+  const int per = P > 64 ? P / 64 : 1;  // leaves per lane (1, 2 or 4)
+  const int last = nblk - 1;            // loads clamp to the last leaf; selects zero-pad
+  double a[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const double* w = v + r * stride;
+    if (per == 1) {
+      const double x0 = w[min(lane, last)];
+      a[r] = lane < nblk ? x0 : 0.0;
+    } else if (per == 2) {
+      const int b = 2 * lane;
+      const double x0 = w[min(b, last)], x1 = w[min(b + 1, last)];
+      a[r] = (b < nblk ? x0 : 0.0) + (b + 1 < nblk ? x1 : 0.0);
+    } else {
+      const int b = 4 * lane;
+      const double x0 = w[min(b, last)], x1 = w[min(b + 1, last)];
+      const double x2 = w[min(b + 2, last)], x3 = w[min(b + 3, last)];
+      a[r] = ((b < nblk ? x0 : 0.0) + (b + 1 < nblk ? x1 : 0.0)) +
+             ((b + 2 < nblk ? x2 : 0.0) + (b + 3 < nblk ? x3 : 0.0));
+    }
+  }
+  wave_tree_sum3_last(a[0], a[1], a[2]);
+  s0 = lane_d(a[0], 63);
+  s1 = lane_d(a[1], 63);
+  s2 = lane_d(a[2], 63);
+}
+
 struct SiftStats {
   double sq_new, sq_diff, sq_old, std_sum, old_max, old_min;
 };
 
-// Not-a-knot envelope of side `side` at samples i and i+1 (bits p and p+1 of 64-sample
-// block b).  The interval comes from the extrema masks of the detection that produced the
-// knots (true extrema at or before the sample plus the left knots); when mirrored knots
-// straddle the sample (only near the trace ends) it is walked to the enclosing one.
+// Wave-uniform description of one not-a-knot side for the envelope pass.
+struct SideEval {
+  int k0;   // first knot slot
+  int nl;   // left (mirrored) knots before the first true knot
+  int nk2;  // knots - 2: the last interval index
+  int xf;   // positions of the first and last true knot: a sample in [xf, xl) lies
+  int xl;   // between two true knots, in the interval the extrema masks give
+};
+
 template <int WG>
-__device__ __forceinline__ void envelope_pair(const Tile& t, const SiftShared<WG>& sh, int side,
-                                              const Work& wk, int b, int p, int i, int tb,
-                                              double& e0, double& e1) {
-  const int k0 = sh.k0[side], nk = sh.nk[side];
-  const uint64_t m = side == 0 ? wk.mmax[b] : wk.mmin[b];
-  const int pre = side == 0 ? wk.pmax[b] : wk.pmin[b];
+__device__ __forceinline__ SideEval side_eval(const SiftShared<WG>& sh, int side, int tb) {
+  SideEval e;
+  e.k0 = uni(sh.k0[side]);
+  e.nl = tb - e.k0;
+  e.nk2 = uni(sh.nk[side]) - 2;
+  e.xf = uni(sh.xf[side]);
+  e.xl = uni(sh.xl[side]);
+  return e;
+}
+
+// Not-a-knot envelope of one side at samples i and i+1 (bits p and p+1 of 64-sample block
+// b).  The interval comes from the extrema masks of the detection that produced the knots:
+// true extrema at or before the sample plus the left knots.  Only samples before the
+// first or after the last true knot (where mirrored knots may straddle them) walk to the
+// enclosing interval; for every other sample the walk would not move.
+template <int SIDE>
+__device__ __forceinline__ void envelope_pair(const Tile& t, const SideEval& s, const Work& wk,
+                                              int b, int p, int i, double& e0, double& e1) {
+  const uint64_t m = SIDE == 0 ? wk.mmax[b] : wk.mmin[b];
+  const int pre = SIDE == 0 ? wk.pmax[b] : wk.pmin[b];
   const uint64_t le_mask = (uint64_t(2) << p) - 1;  // bits 0..p (p is even, so <= 62)
   const int le0 = pre + __popcll(m & le_mask);
   const int le1 = le0 + (int)((m >> (p + 1)) & 1);  // p is even, so p + 1 <= 63
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     const int ii = i + q;
-    int j = (tb - k0) + (q ? le1 : le0) - 1;
-    j = max(0, min(j, nk - 2));
-    int xa = t.X[k0 + j], xb = t.X[k0 + j + 1];
-    if (!((xa <= ii || j == 0) && (xb > ii || j == nk - 2))) {
-      while (j < nk - 2 && t.X[k0 + j + 1] <= ii) ++j;
-      while (j > 0 && t.X[k0 + j] > ii) --j;
-      xa = t.X[k0 + j];
-      xb = t.X[k0 + j + 1];
+    int j = s.nl + (q ? le1 : le0) - 1;
+    if (!(ii >= s.xf && ii < s.xl)) {
+      j = max(0, min(j, s.nk2));
+      while (j < s.nk2 && t.X[s.k0 + j + 1] <= ii) ++j;
+      while (j > 0 && t.X[s.k0 + j] > ii) --j;
     }
-    const int jj = k0 + j;
-    const double ih = t.H[jj];
+    const int jj = s.k0 + j;
+    const int xa = t.X[jj], xb = t.X[jj + 1];
+    const double B = double(ii - xa) * t.H[jj];
+    const double A = 1.0 - B;
     const double hh = double(xb - xa);
-    const double A = double(xb - ii) * ih;
-    const double B = double(ii - xa) * ih;
     // A y0 + B y1 + h^2/6 ((A^3 - A) M0 + (B^3 - B) M1), with fused multiply-adds
     const double ca = __builtin_fma(A * A, A, -A), cb = __builtin_fma(B * B, B, -B);
     const double curv = __builtin_fma(ca, t.M[jj], cb * t.M[jj + 1]);
@@ -1083,26 +1196,27 @@ __device__ __forceinline__ void envelope_pair(const Tile& t, const SiftShared<WG
 }
 
 // One 128-sample block c of the envelope pass: both envelopes at samples i and i+1 of this
-// lane, the IMF update and the block's three stop-statistic sums (lane 63 stores them).
+// lane, the IMF update, and the lane's pair sums of the three stop statistics (in sq).
 // NAK: both sides are known to be not-a-knot.
 template <int WG, bool NAK>
 __device__ __forceinline__ void envelope_block(const Work& wk, const Tile& t,
                                                const SiftShared<WG>& sh, int N, int c, int lane,
-                                               int tb0, int tb1, int kind0, int kind1,
-                                               double& nmax, double& nmin) {
+                                               const SideEval& s0, const SideEval& s1,
+                                               int kind0, int kind1, double* sq, double& nmax,
+                                               double& nmin) {
   const int i = c * 128 + 2 * lane;
-  double ln = 0.0, ld = 0.0, ls = 0.0;
+  sq[0] = sq[1] = sq[2] = 0.0;
   if (i < N) {
     const int b = 2 * c + (lane >> 5), p = (2 * lane) & 63;
     double emax0, emax1, emin0, emin1;
     if (NAK || kind0 == kNotAKnot) {
-      envelope_pair(t, sh, 0, wk, b, p, i, tb0, emax0, emax1);
+      envelope_pair<0>(t, s0, wk, b, p, i, emax0, emax1);
     } else {
       emax0 = eval_small(t, sh.k0[0], kind0, sh.p3[0], i);
       emax1 = eval_small(t, sh.k0[0], kind0, sh.p3[0], i + 1);
     }
     if (NAK || kind1 == kNotAKnot) {
-      envelope_pair(t, sh, 1, wk, b, p, i, tb1, emin0, emin1);
+      envelope_pair<1>(t, s1, wk, b, p, i, emin0, emin1);
     } else {
       emin0 = eval_small(t, sh.k0[1], kind1, sh.p3[1], i);
       emin1 = eval_small(t, sh.k0[1], kind1, sh.p3[1], i + 1);
@@ -1112,60 +1226,80 @@ __device__ __forceinline__ void envelope_block(const Work& wk, const Tile& t,
     const double nw0 = __builtin_fma(-0.5, emax0 + emin0, old0);
     const double d0 = nw0 - old0;
     const double r0 = d0 * recip(nw0);
-    ln = nw0 * nw0;
-    ld = d0 * d0;
-    ls = r0 * r0;
-    nmax = fmax(nmax, nw0);
-    nmin = fmin(nmin, nw0);
-    wk.imf[i] = nw0;
+    sq[0] = nw0 * nw0;
+    sq[1] = d0 * d0;
+    sq[2] = r0 * r0;
+    double hi = nw0, lo = nw0;
     if (i + 1 < N) {
       const double nw1 = __builtin_fma(-0.5, emax1 + emin1, old1);
       const double d1 = nw1 - old1;
       const double r1 = d1 * recip(nw1);
-      ln = __builtin_fma(nw1, nw1, ln);
-      ld = __builtin_fma(d1, d1, ld);
-      ls = __builtin_fma(r1, r1, ls);
-      nmax = fmax(nmax, nw1);
-      nmin = fmin(nmin, nw1);
+      sq[0] = __builtin_fma(nw1, nw1, sq[0]);
+      sq[1] = __builtin_fma(d1, d1, sq[1]);
+      sq[2] = __builtin_fma(r1, r1, sq[2]);
+      hi = fmax(nw0, nw1);
+      lo = fmin(nw0, nw1);
       wk.imf[i + 1] = nw1;
     }
-  }
-  ln = wave_tree_sum_last(ln);
-  ld = wave_tree_sum_last(ld);
-  ls = wave_tree_sum_last(ls);
-  if (lane == 63) {
-    wk.bsum[c] = ln;
-    wk.bsum[wk.nbcap + c] = ld;
-    wk.bsum[2 * wk.nbcap + c] = ls;
+    wk.imf[i] = nw0;
+    nmax = fmax(nmax, hi);
+    nmin = fmin(nmin, lo);
   }
 }
 
+// Sub-blocks of 128 samples per envelope-pass superblock: a function of N alone, so the
+// association of the stop-statistic sums is the same for every launch shape.
+__device__ __forceinline__ int sub_blocks(int nblk2) { return min(4, max(1, nblk2 >> 2)); }
+
 // One sift: evaluate both envelopes at every sample, subtract the mean, accumulate the
 // PyEMD stop statistics.  The IMF in LDS is updated in place.  Each lane takes two
-// adjacent samples of a 128-sample block; the pair is added first, then the 64-lane
-// tree, then the tree over blocks -- the same perfect binary tree over next_pow2(N)
-// samples (zero-padded) as the 64-sample blocks of the other reductions.
+// adjacent samples of a 128-sample block and adds them; a superblock of K = sub_blocks()
+// consecutive 128-sample blocks is one wave's unit, whose lane pair sums are added as a
+// perfect tree over the K blocks, then over the 64 lanes; the superblock sums are then
+// added as a perfect tree over superblocks (zero-padded).  Every sum therefore has one
+// association for a given N, whatever the workgroup size, tile or queue.
 // Sift n's imf_old is the IMF that sift n-1 left, bit for bit (the detection in between
-// does not touch it), so its sum of squares (the same tree as sq_new) and its range come
-// from sh.carry[(n-1)&1] instead of being recomputed; the IMF's start pass fills carry[0]
-// for n == 1.
+// does not touch it), so its sum of squares and its range come from sh.carry[(n-1)&1]
+// instead of being recomputed; the IMF's start pass fills carry[0] for n == 1.
 template <int WG>
 __device__ __forceinline__ SiftStats envelope_update(const Work& wk, const Tile& t,
                                                      SiftShared<WG>& sh, int N,
                                                      const DetectResult& D, int nb, int n) {
   constexpr int NW = WG / 64;
-  const int tid = fresh_tid(), lane = tid & 63, w = tid >> 6;
+  const int tid = fresh_tid(), lane = tid & 63, w = uni(tid >> 6);
   const int nblk2 = ceil_div(N, 128);
-  const int tb0 = nb, tb1 = 3 * nb + D.nm;
+  const int K = sub_blocks(nblk2);
+  const int nsb = ceil_div(nblk2, K);
   const int kind0 = uni(sh.kind[0]), kind1 = uni(sh.kind[1]);
+  const SideEval s0 = side_eval(sh, 0, nb), s1 = side_eval(sh, 1, 3 * nb + D.nm);
   double nmax = -INFINITY, nmin = INFINITY;
-  // the common case (both envelopes not-a-knot) gets a loop without the small-kind paths
-  if (kind0 == kNotAKnot && kind1 == kNotAKnot) {
-    for (int c = w; c < nblk2; c += NW)
-      envelope_block<WG, true>(wk, t, sh, N, c, lane, tb0, tb1, kind0, kind1, nmax, nmin);
-  } else {
-    for (int c = w; c < nblk2; c += NW)
-      envelope_block<WG, false>(wk, t, sh, N, c, lane, tb0, tb1, kind0, kind1, nmax, nmin);
+  const bool nak = kind0 == kNotAKnot && kind1 == kNotAKnot;
+  for (int sb = w; sb < nsb; sb += NW) {
+    double a[3] = {0.0, 0.0, 0.0}, z[3] = {0.0, 0.0, 0.0};  // blocks 0+1 and 2+3
+    for (int q = 0; q < K; ++q) {
+      const int c = sb * K + q;
+      if (c >= nblk2) break;
+      double v[3];
+      // the common case (both envelopes not-a-knot) gets a loop without the small kinds
+      if (nak)
+        envelope_block<WG, true>(wk, t, sh, N, c, lane, s0, s1, kind0, kind1, v, nmax, nmin);
+      else
+        envelope_block<WG, false>(wk, t, sh, N, c, lane, s0, s1, kind0, kind1, v, nmax, nmin);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        if (q == 0) a[k] = v[k];
+        else if (q == 1) a[k] += v[k];
+        else if (q == 2) z[k] = v[k];
+        else z[k] += v[k];
+      }
+    }
+    double s_new = a[0] + z[0], s_diff = a[1] + z[1], s_std = a[2] + z[2];
+    wave_tree_sum3_last(s_new, s_diff, s_std);
+    if (lane == 63) {
+      wk.bsum[sb] = s_new;
+      wk.bsum[wk.nbcap + sb] = s_diff;
+      wk.bsum[2 * wk.nbcap + sb] = s_std;
+    }
   }
   nmax = wave_max(nmax);
   nmin = wave_min(nmin);
@@ -1175,9 +1309,7 @@ __device__ __forceinline__ SiftStats envelope_update(const Work& wk, const Tile&
   }
   __syncthreads();
   SiftStats st;
-  st.sq_new = blocks_tree(wk.bsum, nblk2);
-  st.sq_diff = blocks_tree(wk.bsum + wk.nbcap, nblk2);
-  st.std_sum = blocks_tree(wk.bsum + 2 * wk.nbcap, nblk2);
+  blocks_tree3(wk.bsum, wk.nbcap, nsb, st.sq_new, st.sq_diff, st.std_sum);
   double gnx = sh.red[0][2], gnn = sh.red[0][3];
 #pragma unroll
   for (int q = 1; q < NW; ++q) {
@@ -1221,15 +1353,17 @@ __device__ __forceinline__ void build_envelopes(const Tile& t, SiftShared<WG>& s
   if (threadIdx.x < 64) {
     __builtin_amdgcn_s_setprio(2);  // the mirror is the longest single-wave step
     int bad = sh.bad;
-    int k0[2], nk[2], kind[2];
+    int k0[2], nk[2], kind[2], xf[2], xl[2];
     double p3[2][4];
-    mirror(t, N, D.nm, D.nn, P.nbsym, wk.imf[0], wk.imf[N - 1], k0, nk, kind, bad, p3);
+    mirror(t, N, D.nm, D.nn, P.nbsym, wk.imf[0], wk.imf[N - 1], k0, nk, kind, bad, p3, xf, xl);
     if (threadIdx.x == 0) {
       sh.bad = bad;
       for (int q = 0; q < 2; ++q) {
         sh.k0[q] = k0[q];
         sh.nk[q] = nk[q];
         sh.kind[q] = kind[q];
+        sh.xf[q] = xf[q];
+        sh.xl[q] = xl[q];
         for (int r = 0; r < 4; ++r) sh.p3[q][r] = p3[q][r];
       }
       const int m0 = kind[0] == kNotAKnot ? nk[0] - 2 : 0;
@@ -1274,7 +1408,7 @@ template <int WG>
 __device__ __forceinline__ void start_imf(const SignalIO& io, SiftShared<WG>& sh, const Work& wk,
                                           double* acc, bool have_acc, bool add) {
   constexpr int NW = WG / 64;
-  const int tid = fresh_tid(), lane = tid & 63, w = tid >> 6;
+  const int tid = fresh_tid(), lane = tid & 63, w = uni(tid >> 6);
   const int N = io.N, nblk = ceil_div(N, 64);
   // block sums in the fourth row of the work area's sums: the end-condition reduction
   // that precedes this pass may still be reading the first
@@ -1337,7 +1471,7 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
                                           const Work& wk, const Tile& tl, const Tile& tg,
                                           const SiftParams& P) {
   constexpr int NW = WG / 64;
-  const int tid = fresh_tid(), lane = tid & 63, w = tid >> 6;
+  const int tid = fresh_tid(), lane = tid & 63, w = uni(tid >> 6);
   const int N = io.N;
   const int nblk = ceil_div(N, 64);
   const int nb = P.nbsym;
@@ -1642,7 +1776,7 @@ template <int WG>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void emd_envelope_kernel(EnvBatch e, SiftParams P) {
   extern __shared__ double dyn_lds[];
   __shared__ SiftShared<WG> sh;
-  const int tid = fresh_tid(), lane = tid & 63, w = tid >> 6;
+  const int tid = fresh_tid(), lane = tid & 63, w = uni(tid >> 6);
   constexpr int NW = WG / 64;
   const WgMem m = workgroup_mem(dyn_lds, e.spill, e.spill_stride, P);
   const int bidx = blockIdx.x;
