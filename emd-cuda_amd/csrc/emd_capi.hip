@@ -169,7 +169,9 @@ int make_shape(const emd_handle* h, int wg, int tile_req, int max_n, Shape* s) {
   return EMD_OK;
 }
 
-emdk::SiftParams params(const emd_handle* h, const Shape& s) {
+// zero_fill: write zeros into each trace's rows at and past n_imfs (the padded layout,
+// whose reference counterpart is a zero-initialised array); the ragged layout leaves them.
+emdk::SiftParams params(const emd_handle* h, const Shape& s, int zero_fill) {
   emdk::SiftParams P;
   P.max_extrema = h->cfg.max_extrema;
   P.nbsym = h->cfg.nbsym;
@@ -179,7 +181,7 @@ emdk::SiftParams params(const emd_handle* h, const Shape& s) {
   P.spill_slots = s.spill_slots;
   P.spill_rows = s.spill_rows;
   P.max_n = s.max_n;
-  P.zero_fill = 1;
+  P.zero_fill = zero_fill;
   return P;
 }
 
@@ -319,11 +321,12 @@ struct Plan {
   size_t spill_bytes = 0;  // stride x resident workgroups
 };
 
-static int make_plan(const emd_handle* h, int B, int max_n, const emd_launch& L, Plan* pl) {
+static int make_plan(const emd_handle* h, int B, int max_n, const emd_launch& L, int zero_fill,
+                     Plan* pl) {
   int rc = make_shape(h, L.threads, L.tile_slots > 0 ? L.tile_slots : h->cfg.tile_slots, max_n,
                       &pl->s);
   if (rc) return rc;
-  pl->P = params(h, pl->s);
+  pl->P = params(h, pl->s, zero_fill);
   pl->stride = pl->s.spill_slots
                    ? emdk::global_region_bytes(pl->s.spill_slots, pl->s.spill_rows)
                    : 0;
@@ -441,7 +444,7 @@ static int run_common(emd_handle* h, const double* values, const int64_t* offset
   if (B == 0) return EMD_OK;
   if (L.traces < 0) return fail(EMD_ERR_VALUE, "launch.traces must be >= 0, got %d", L.traces);
   Plan pl;
-  if ((rc = make_plan(h, B, max_n, L, &pl))) return rc;
+  if ((rc = make_plan(h, B, max_n, L, offsets == nullptr, &pl))) return rc;
   if (pl.spill_bytes && (rc = ensure_spill(h, pl.spill_bytes))) return rc;
   const emdk::Batch b = make_batch(values, offsets, N, order, L.traces > 0 ? L.traces : B,
                                    imfs, out_offsets, n_imfs, overflow, debug_n, debug_extno,
@@ -515,7 +518,7 @@ int emd_run_ragged_noalloc(emd_handle* h, const double* values, const int64_t* o
   if (B == 0) return EMD_OK;
   if (!h->static_scratch) return fail(EMD_ERR_RUNTIME, "static scratch address unknown");
   Plan pl;
-  if ((rc = make_plan(h, B, max_n, L, &pl))) return rc;
+  if ((rc = make_plan(h, B, max_n, L, 0, &pl))) return rc;
   const size_t room = kStaticScratchBytes - kCounterBytes;
   if (pl.stride > 0) {
     if ((size_t)pl.stride > room)
@@ -549,7 +552,7 @@ int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double
   Shape s;
   int rc = make_shape(h, 256, h->cfg.tile_slots, N, &s);
   if (rc) return rc;
-  const emdk::SiftParams P = params(h, s);
+  const emdk::SiftParams P = params(h, s, 1);
   const int mbuf = h->cfg.max_extrema + 2 * h->cfg.nbsym;
   const int64_t stride =
       s.spill_slots ? emdk::global_region_bytes(s.spill_slots, s.spill_rows) : 0;
@@ -619,7 +622,7 @@ int emd_chunk_graph_create(emd_handle* h, const emd_chunk* c, emd_graph** out) {
   if (c->launch.traces < 0)
     return fail(EMD_ERR_VALUE, "launch.traces must be >= 0, got %d", c->launch.traces);
   Plan pl;
-  if ((rc = make_plan(h, c->B, max_n, c->launch, &pl))) return rc;
+  if ((rc = make_plan(h, c->B, max_n, c->launch, c->offsets == nullptr, &pl))) return rc;
   emd_graph* g = new emd_graph();
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&g->counters), 256);
   if (e == hipSuccess && pl.spill_bytes)

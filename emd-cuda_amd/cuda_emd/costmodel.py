@@ -242,6 +242,24 @@ class CostModel:
             tile = min(tile, cap)
         return Plan(order, tile)
 
+    def order(self, feats, lengths):
+        """The queue order of plan() alone, on the device (no host read): longest predicted
+        sifts x N first, or N alone when the sift prediction does not rank (then feats may
+        be None)."""
+        torch = _torch()
+        L = lengths if isinstance(lengths, torch.Tensor) else torch.from_numpy(
+            np.asarray(lengths, dtype=np.float64))
+        L = L.to(self.device, torch.float64)
+        if not self.orders_by_sifts:
+            return torch.argsort(L, descending=True, stable=True).to(torch.int32)
+        y = self.forward_std(feats).double()
+        if self._tstd is None:
+            self._tstd = (torch.tensor(self.tsd, dtype=torch.float64, device=self.device),
+                          torch.tensor(self.tmu, dtype=torch.float64, device=self.device))
+        tsd, tmu = self._tstd
+        cost = torch.expm1(y[:, 0] * tsd[0] + tmu[0]).clamp_min(1.0) * L
+        return torch.argsort(cost, descending=True, stable=True).to(torch.int32)
+
     # -- persistence -------------------------------------------------------------------------
     def state(self) -> dict:
         return {"params": self.params.cpu().numpy(), "norm": self.norm.cpu().numpy(),

@@ -220,8 +220,8 @@ class EMDBatchLauncher:
                    wgs_per_cu=0, persistent=True, out_offsets=None) -> tuple:
         """Decompose CSR traces (values, int64 offsets [B+1]).  ``order`` (int32 queue
         order) never changes outputs.  Returns (imfs flat, out_offsets [B] int64, n_imfs,
-        overflow, debug_n, debug_extno, debug_nzm, stats); row k of trace s is
-        imfs[out_offsets[s] + k*len(s) : ... + len(s)]."""
+        overflow, debug_n, debug_extno, debug_nzm, stats); row k < n_imfs[s] of trace s is
+        imfs[out_offsets[s] + k*len(s) : ... + len(s)] (rows past n_imfs are scratch)."""
         torch = _torch()
         v = _as_device_f64(values, "values")
         off = offsets if isinstance(offsets, torch.Tensor) else torch.from_numpy(

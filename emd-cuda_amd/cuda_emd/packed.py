@@ -288,9 +288,9 @@ class PipelinedEMD:
     """A stream of batches through the learned-packed persistent kernel, with consecutive
     batches overlapped.
 
-    Each ``submit`` plans its batch (features -> MLP -> queue order + LDS tile; the FIFO
-    order and ``tile_slots`` without a cost model) on a planning stream, then launches the
-    persistent sift on the next of ``depth`` lanes.  A lane is a launcher handle (its own
+    Each ``submit`` plans its batch (features -> MLP -> queue order, and on the first batch
+    the LDS tile; the FIFO order and ``tile_slots`` without a cost model) on a planning
+    stream, then launches the persistent sift on the next of ``depth`` lanes.  A lane is a launcher handle (its own
     queue counter and global knot tiles) with its own stream, so a batch's sift takes the
     CU slots the previous batch's sift frees as its queue drains, instead of leaving them
     idle in that launch's tail.  The plan reads only its batch, so it also runs while the
@@ -319,16 +319,26 @@ class PipelinedEMD:
         self._events = []
         self._done = {}
         self._k = 0
+        self._tile = None   # LDS tile the cost model sized on the first batch
         self.last_tile = tile_slots
 
     def _plan(self, values, offsets_dev, lengths_dev):
+        """Queue order and LDS tile of one batch.  The first batch's plan sizes the tile
+        from the predicted peak extrema (one host read); later batches keep that tile and
+        compute only the order, on the device, so no step waits for its plan on the host.
+        When the model's sift prediction does not rank traces the order is by length
+        alone and needs no features."""
         from . import costmodel
         cm = self.cost_model
         if cm is None:
             return None, self.tile_slots
+        if self.tile_slots or self._tile is not None:
+            f = costmodel.features(values, offsets_dev) if cm.orders_by_sifts else None
+            return cm.order(f, lengths_dev), self.tile_slots or self._tile
         f = costmodel.features(values, offsets_dev)
         plan = cm.plan(f, lengths_dev, nbsym=self.config.nbsym, quantile=0.995)
-        return plan.order, self.tile_slots or plan.tile_slots
+        self._tile = plan.tile_slots
+        return plan.order, plan.tile_slots
 
     def submit(self, values, offsets, offsets_dev=None, lengths_dev=None) -> tuple:
         """Queue one CSR batch (float64 device ``values``, int64 ``offsets`` [B+1]).

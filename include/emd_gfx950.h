@@ -86,8 +86,10 @@ int emd_run_padded(emd_handle* h, const double* signals, int B, int N, double* i
  * Outputs are indexed by trace, so the order never changes them.  max_n must be >= the
  * length of every queued trace; a longer one is reported with overflow = -1 and no rows.
  * imfs row k of trace s starts at imfs + out_offsets[s] + k*len(s) (out_offsets NULL =
- * max_imfs * offsets[s]).  No reference counterpart (the reference needs one shared N);
- * its per-trace semantics are emd_batch_kernel's. */
+ * max_imfs * offsets[s]); a trace's rows at and past n_imfs[s] are scratch and hold
+ * unspecified values (only the padded layout of emd_run_padded is zero-filled, like the
+ * reference's zero-initialised output).  No reference counterpart (the reference needs one
+ * shared N); its per-trace semantics are emd_batch_kernel's. */
 int emd_run_ragged(emd_handle* h, const double* values, const int64_t* offsets, int B,
                    int max_n, const int* order, const emd_launch* launch, double* imfs,
                    const int64_t* out_offsets, int* n_imfs, int* overflow, int* debug_n,

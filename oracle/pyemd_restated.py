@@ -24,6 +24,16 @@ reference-only rules that have no PyEMD counterpart are available as options:
 * a signal with ``extNo > 2`` but no maximum or no minimum is treated as a trend, as the
   reference does (kernels.py:934); PyEMD raises IndexError there.
 
+``mode='reference'`` switches four rules to the reference kernel's own (for measuring how
+often the two semantics part ways; the HIP kernels implement PyEMD's):
+
+* extrema and zero crossings by the reference's scan (find_extrema_reference): its
+  parallel path when no sample in [1, N-2] is zero or flat, else its serial path
+  (kernels.py:115-204, 231-349);
+* a 3-knot envelope is the interpolating parabola (kernels.py:711-718);
+* the residue is updated by subtraction, residue -= imf, and the end condition, the trend
+  drop and the appended residue use that running residue (kernels.py:994-1075).
+
 Near ties: every stop statistic within 1e-12 (relative) of its threshold, and every
 sign test (a sample or first difference within 1e-12 * max|imf| of zero, after the raw
 input) is logged in ``EMDResult.near_ties``; only there may rounding-level differences
@@ -116,6 +126,65 @@ def find_extrema_simple(S: np.ndarray):
             indmin = np.sort(np.append(indmin, np.asarray(imin, np.int64)))
 
     return indmax.astype(np.int64), indmin.astype(np.int64), int(len(indzer))
+
+
+def find_extrema_reference(S: np.ndarray):
+    """The reference kernel's extrema and zero-crossing rules (reference mode).
+
+    Parallel path (kernels.py:231-349), used when no sample i in [1, N-2] has S[i] == 0,
+    S[i] == S[i-1] or S[i+1] == S[i]: strict extrema by d1*d2 < 0, zero crossings =
+    #(S[i] != 0 and S[i]*S[i+1] < 0), so a zero at sample 0 or N-1 is not counted.
+    Otherwise the serial path (kernels.py:115-204): a run of exact zeros counts once when
+    it ends (a run reaching the end is never counted); a plateau is found at its first
+    sample i >= 1 (so one starting at sample 0 is never seen), dropped when it reaches the
+    end, and its extremum sits at floor((first + last) / 2).
+    Returns (indmax, indmin, nzm)."""
+    S = np.asarray(S, dtype=np.float64)
+    N = S.size
+    if N < 2:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64), 0
+    inner = np.arange(1, N - 1)
+    d = np.diff(S)
+    trigger = np.any((S[inner] == 0) | (d[inner - 1] == 0) | (d[inner] == 0)) if N > 2 else False
+    if not trigger:
+        d1, d2 = d[:-1], d[1:]
+        ext = d1 * d2 < 0
+        indmax = np.nonzero(ext & (d1 > 0))[0] + 1
+        indmin = np.nonzero(ext & (d1 < 0))[0] + 1
+        nz = int(np.count_nonzero((S[:-1] != 0) & (S[:-1] * S[1:] < 0)))
+        return indmax.astype(np.int64), indmin.astype(np.int64), nz
+    nz = 0
+    in_run = False
+    for i in range(N):
+        z = S[i] == 0.0
+        if z and not in_run:
+            in_run = True
+        if not z and in_run:
+            nz += 1
+            in_run = False
+        if not z and i + 1 < N and S[i] * S[i + 1] < 0.0:
+            nz += 1
+    imax, imin = [], []
+    i = 1
+    while i < N - 1:
+        dprev = S[i] - S[i - 1]
+        dnext = S[i + 1] - S[i]
+        if dprev * dnext < 0.0:
+            (imin if dprev < 0.0 else imax).append(i)
+        elif dnext == 0.0 and dprev != 0.0:
+            j = i
+            while j + 1 < N and S[j + 1] == S[i]:
+                j += 1
+            if j + 1 < N:
+                d_after = S[j + 1] - S[j]
+                mid = (i + j) // 2
+                if dprev > 0.0 and d_after < 0.0:
+                    imax.append(mid)
+                elif dprev < 0.0 and d_after > 0.0:
+                    imin.append(mid)
+            i = j
+        i += 1
+    return np.asarray(imax, np.int64), np.asarray(imin, np.int64), nz
 
 
 # ---------------------------------------------------------------------------------------
@@ -260,7 +329,7 @@ def cubic_spline_3pts(x, y, t):
     return np.append(q1, q2)
 
 
-def spline_points(T: np.ndarray, extrema: np.ndarray) -> np.ndarray:
+def spline_points(T: np.ndarray, extrema: np.ndarray, mode: str = "pyemd") -> np.ndarray:
     """PyEMD ``EMD.spline_points`` for spline_kind='cubic': evaluate on the samples of T
     inside [first knot, last knot]; more than 3 knots -> ``interp1d(kind='cubic')``
     (scipy ``make_interp_spline(k=3)``, not-a-knot), 3 knots -> ``cubic_spline_3pts``.
@@ -274,6 +343,13 @@ def spline_points(T: np.ndarray, extrema: np.ndarray) -> np.ndarray:
     if nk > 3:
         return interp1d(extrema[0], extrema[1], kind="cubic")(t)
     if nk == 3:
+        if mode == "reference":
+            # the reference's 3-knot envelope: the interpolating parabola, Lagrange form
+            (x0, x1, x2), (y0, y1, y2) = extrema[0], extrema[1]
+            l0 = (t - x1) * (t - x2) / ((x0 - x1) * (x0 - x2))
+            l1 = (t - x0) * (t - x2) / ((x1 - x0) * (x1 - x2))
+            l2 = (t - x0) * (t - x1) / ((x2 - x0) * (x2 - x1))
+            return y0 * l0 + y1 * l1 + y2 * l2
         return cubic_spline_3pts(extrema[0], extrema[1], t)
     if nk == 2:
         x0, x1 = extrema[0]
@@ -368,14 +444,20 @@ class EMDResult:
 
 
 def emd(S, max_imf: int = -1, max_iteration: int = 1000, nbsym: int = 2,
-        max_extrema: int | None = None, debug_slots: int = 16) -> EMDResult:
+        max_extrema: int | None = None, debug_slots: int = 16,
+        mode: str = "pyemd") -> EMDResult:
     """PyEMD ``EMD.emd`` with the default stopping rule (FIXE = FIXE_H = 0).
 
     T is always arange(N) (PyEMD replaces T in 'simple' mode).  Reference counterpart:
     emd_batch_kernel (kernels.py:834-1080); debug records follow kernels.py:901-928,
     972-979 (sift count at convergence, = max_iteration with extno/nzm -1 when capped,
-    -1 for slots never reached).
+    -1 for slots never reached).  ``mode='reference'`` applies the reference kernel's own
+    extrema, 3-knot and running-residue rules (module docstring).
     """
+    if mode not in ("pyemd", "reference"):
+        raise ValueError(f"mode must be 'pyemd' or 'reference', got {mode!r}")
+    ref = mode == "reference"
+    find = find_extrema_reference if ref else find_extrema_simple
     S = np.asarray(S, dtype=np.float64)
     N = S.size
     T = np.arange(N, dtype=np.float64)
@@ -394,7 +476,7 @@ def emd(S, max_imf: int = -1, max_iteration: int = 1000, nbsym: int = 2,
 
     def extrema(x, where=None):
         nonlocal overflow, peak
-        imax, imin, nz = find_extrema_simple(x)
+        imax, imin, nz = find(x)
         if where is not None and x.size:
             # sign tests at a rounding-level distance from zero: a sample (zero crossing)
             # or a first difference (extremum) within 1e-12 * max|x| of 0
@@ -417,8 +499,9 @@ def emd(S, max_imf: int = -1, max_iteration: int = 1000, nbsym: int = 2,
     extNo = -1
     finished = False
     cap_hit = 0
+    run_res = S.copy()  # reference mode: the running residue
     while not finished:
-        residue = S - np.sum(IMF[:imfNo], axis=0)
+        residue = run_res if ref else S - np.sum(IMF[:imfNo], axis=0)
         imf = residue.copy()
         n = 0
         while True:
@@ -432,8 +515,8 @@ def emd(S, max_imf: int = -1, max_iteration: int = 1000, nbsym: int = 2,
             extNo = len(imax) + len(imin)
             if extNo > 2 and len(imax) >= 1 and len(imin) >= 1:
                 max_k, min_k = prepare_points_simple(imf, imax, imin, nbsym)
-                max_env = spline_points(T, max_k)
-                min_env = spline_points(T, min_k)
+                max_env = spline_points(T, max_k, mode)
+                min_env = spline_points(T, min_k, mode)
                 total_sifts += 1
                 knot_sum += max_k.shape[1] + min_k.shape[1]
                 mean = 0.5 * (max_env + min_env)
@@ -452,16 +535,24 @@ def emd(S, max_imf: int = -1, max_iteration: int = 1000, nbsym: int = 2,
                 break
         IMF = np.vstack((IMF, imf.copy()))
         imfNo += 1
-        if end_condition(S, IMF, ties, (imfNo - 1, n)):
+        if ref:
+            run_res = run_res - imf
+            rng = np.max(run_res) - np.min(run_res)
+            if rng < RANGE_THR or np.sum(np.abs(run_res)) < TOTAL_POWER_THR:
+                finished = True
+                break
+        elif end_condition(S, IMF, ties, (imfNo - 1, n)):
             finished = True
             break
         if imfNo == max_imf:
             cap_hit = 0 if finished else 1
             finished = True
             break
-    if extNo <= 2:
+    if extNo <= 2 and IMF.shape[0] > 0:
+        if ref:
+            run_res = run_res + IMF[-1]  # the reference undoes the trend's subtraction
         IMF = IMF[:-1]
-    residue = S - np.sum(IMF, axis=0)
+    residue = run_res if ref else S - np.sum(IMF, axis=0)
     if not np.allclose(residue, 0):
         IMF = np.vstack((IMF, residue))
     return EMDResult(IMF, IMF.shape[0], overflow, dbg_n, dbg_e, dbg_z, total_sifts, peak,

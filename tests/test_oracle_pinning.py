@@ -259,3 +259,75 @@ def test_oracle_residue_is_signal_minus_sequential_sum():
     half = (k + 1) // 2
     pair = rows[:half].sum(axis=0) + rows[half:].sum(axis=0)
     assert not np.array_equal(x - pair, x - seq)
+
+
+# ---------------------------------------------------------------------------------------
+# Reference mode (the reference kernel's own rules), each on a constructed case whose
+# answer follows from the rule as written in reference kernels.py, and differs from
+# PyEMD's where the two disagree.
+# ---------------------------------------------------------------------------------------
+def test_reference_zero_runs_counted_when_they_end():
+    """Serial path (a zero inside the trace triggers it): each run of exact zeros counts
+    once when it ends (kernels.py:126-143), so the trailing run is not counted; PyEMD
+    counts every run.  S: +, 0, 0, -, +, 0 -> runs [1,2] (ends at 3), [5] (never ends);
+    sign changes: (3,4)."""
+    S = np.array([1.0, 0.0, 0.0, -1.0, 2.0, 0.0])
+    assert O.find_extrema_reference(S)[2] == 2
+    assert O.find_extrema_simple(S)[2] == 3
+
+
+def test_reference_parallel_path_ignores_edge_zeros():
+    """No zero or flat sample inside [1, N-2]: the parallel path counts only sign changes
+    with a nonzero left sample (kernels.py:272-275), so zeros at sample 0 and N-1 do not
+    count; PyEMD counts them."""
+    S = np.array([0.0, 1.0, -1.0, 2.0, 0.0])
+    assert O.find_extrema_reference(S)[2] == 2
+    assert O.find_extrema_simple(S)[2] == 4
+
+
+def test_reference_plateau_midpoint_floors():
+    """A plateau over samples 2..5 between a rise and a fall: the reference puts the maximum
+    at floor((2+5)/2) = 3 (kernels.py:191), PyEMD at round(3.5) = 4 (half to even)."""
+    S = np.array([0.0, 1.0, 2.0, 2.0, 2.0, 2.0, 1.0, 0.0])
+    assert list(O.find_extrema_reference(S)[0]) == [3]
+    assert list(O.find_extrema_simple(S)[0]) == [4]
+    # a plateau over 2..4: floor(3) = round(3) = 3 in both
+    S = np.array([0.0, 1.0, 2.0, 2.0, 2.0, 1.0, 0.0])
+    assert list(O.find_extrema_reference(S)[0]) == list(O.find_extrema_simple(S)[0]) == [3]
+
+
+def test_reference_plateau_at_start_is_not_seen():
+    """The serial scan finds a plateau at its first sample i >= 1 with a sloped sample
+    before it, so a plateau starting at sample 0 is never an extremum; a plateau reaching
+    the end is dropped in both."""
+    S = np.array([1.0, 1.0, 0.0, 2.0, 1.0, 1.0])
+    mx, mn, _ = O.find_extrema_reference(S)
+    assert list(mx) == [3] and list(mn) == [2]
+
+
+def test_reference_three_knot_parabola():
+    """3 knots: the reference evaluates the interpolating parabola (kernels.py:711-718):
+    exact at the knots, constant second difference; PyEMD's natural cubic differs."""
+    ext = np.array([[0.0, 3.0, 8.0], [1.0, -2.0, 4.0]])
+    T = np.arange(9, dtype=np.float64)
+    q = O.spline_points(T, ext, mode="reference")
+    assert np.allclose(q[[0, 3, 8]], [1.0, -2.0, 4.0], rtol=0, atol=1e-14)
+    assert np.allclose(np.diff(q, 2), np.diff(q, 2)[0], rtol=0, atol=1e-12)
+    c = O.spline_points(T, ext)
+    assert np.max(np.abs(c - q)) > 1e-3
+
+
+def test_reference_running_residue():
+    """Reference mode: the appended residue is S minus the IMF rows subtracted one at a
+    time (kernels.py:996), bit for bit; PyEMD's is S minus their sum."""
+    from cuda_emd import synth
+    x = synth.seismic_traces(1, 700, seed=9)[0]
+    r = O.emd(x, max_imf=15, mode="reference")
+    res = x.copy()
+    for q in r.imfs[:-1]:
+        res = res - q
+    assert np.array_equal(r.imfs[-1], res)
+    p = O.emd(x, max_imf=15)
+    assert p.n_imfs == r.n_imfs and np.array_equal(p.debug_n, r.debug_n)
+    assert not np.array_equal(p.imfs[-1], r.imfs[-1])
+    assert np.max(np.abs(p.imfs - r.imfs)) < 1e-12 * np.max(np.abs(x))
