@@ -152,11 +152,15 @@ class PackedEMD:
     """
 
     def __init__(self, config: EMDKernelConfig | None = None, cost_model=None,
-                 autotuner: Autotuner | None = None, tile_quantile: float = 0.995):
+                 autotuner: Autotuner | None = None, tile_quantile: float = 0.995,
+                 persistent: bool = True):
+        """persistent=False launches one workgroup per trace in each bucket instead of the
+        persistent queue (the reference's launch model applied per length bucket)."""
         self.config = config or EMDKernelConfig()
         self.cost_model = cost_model
         self.autotuner = autotuner or Autotuner(self.config)
         self.tile_quantile = tile_quantile
+        self.persistent = persistent
         self.launchers = {}
         self.graphs = {}
 
@@ -209,7 +213,8 @@ class PackedEMD:
             spec = _lib.EmdChunkC(
                 _ptr(self.values).value, _ptr(self.offsets).value, 0, len(idx), bmax,
                 _ptr(self.order_bufs[int(b)]).value,
-                _lib.EmdLaunchC(ch.threads, ch.tile_slots, ch.wgs_per_cu, 1, B),
+                _lib.EmdLaunchC(ch.threads, ch.tile_slots, ch.wgs_per_cu,
+                                1 if self.persistent else 0, B),
                 _ptr(self.imfs).value, None, _ptr(self.n_imfs).value, _ptr(self.overflow).value,
                 _ptr(self.debug_n).value, _ptr(self.debug_extno).value,
                 _ptr(self.debug_nzm).value, _ptr(self.stats).value, None, None)
@@ -282,6 +287,17 @@ class PackedEMD:
             self.close()
         except Exception:
             pass
+
+
+class FixedLaunch:
+    """An autotuner stand-in that returns one launch shape for every bucket (for example
+    the reference's: 128 lanes, the capacity-rule LDS tile)."""
+
+    def __init__(self, threads: int = 128, tile_slots: int = 0, wgs_per_cu: int = 0):
+        self.choice = LaunchChoice(threads, tile_slots, wgs_per_cu)
+
+    def choose(self, launcher, values, offsets_host, idx, bucket, tile_hint=0):
+        return self.choice
 
 
 class PipelinedEMD:
