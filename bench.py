@@ -454,6 +454,147 @@ def bench_segy(args):
     print(json.dumps(line), flush=True)
 
 
+def hilbert_work(lengths: np.ndarray, n_imfs: np.ndarray, n_fft: int):
+    """(algorithmic bytes, algorithmic flops) of one Hilbert launch.  Bytes: each IMF sample
+    is read once and its amplitude, phase and frequency written once (32 B).  Flops: per
+    row pair one forward and one inverse complex transform of the row's transform length F
+    (5 F log2 F each, the radix-2 count), per trace in the exact form one more forward
+    transform (its multiplier), 4 per point for the multiplier product, and 12 per sample
+    for the outputs' arithmetic outside atan2 and sqrt."""
+    L = lengths.astype(np.int64)
+    rows = n_imfs.astype(np.float64)
+    samples = float(np.sum(rows * L))
+    pow2 = (L & (L - 1)) == 0
+    if n_fft > 0:
+        F = np.full(len(L), n_fft, np.float64)
+        exact = np.zeros(len(L), bool)
+    elif n_fft < 0:
+        F = 2.0 ** np.ceil(np.log2(np.maximum(L, 1)))
+        exact = np.zeros(len(L), bool)
+    else:
+        exact = (~pow2) & (L > 2)
+        F = np.where(exact, 2.0 ** np.ceil(np.log2(np.maximum(2 * L - 1, 1))), L.astype(float))
+    fft = 5.0 * F * np.log2(np.maximum(F, 2))
+    pairs = np.ceil(rows / 2)
+    flops = np.sum(pairs * (2 * fft + 4 * F) + exact * (rows > 0) * fft) + 12.0 * samples
+    return 32.0 * samples, float(flops)
+
+
+def _hilbert_cpu_one(args):
+    from oracle import hilbert_ref as H
+    rows, dt = args
+    for r in rows:
+        H.instantaneous(r, dt=dt)
+    return len(rows)
+
+
+def bench_hilbert(args):
+    """The Hilbert spectral step on the IMFs of the seismic workload (16384 traces x N=2001,
+    seed 1): the traces are decomposed once (untimed), their IMF rows compacted back to back,
+    and each timed step computes amplitude, phase and frequency of every row (dt = 4 ms)."""
+    import torch
+    from multiprocessing import Pool
+
+    from cuda_emd import EMDKernelConfig, synth
+    from cuda_emd.hilbert import HilbertStep
+    from oracle import hilbert_ref as H
+
+    B, N, dt = args.batch, args.n, 0.004
+    x = synth.seismic_traces(B, N, seed=1)
+    # the CPU-baseline workers fork now, before this process touches the GPU
+    ncpu = min(16, os.cpu_count() or 1)
+    pool = None if args.no_cpu else Pool(ncpu)
+    launcher = EMDKernelConfig.for_device().build()
+    _progress("hilbert: decomposing the input traces")
+    imfs, n_imfs = launcher.run(torch.from_numpy(x).cuda())[:2]
+    row, comp = launcher.compact(imfs, n_imfs)
+    del imfs
+    offsets = torch.arange(0, (B + 1) * N, N, dtype=torch.int64, device="cuda")
+    nim = n_imfs.cpu().numpy()
+    lengths = np.full(B, N, np.int64)
+    res = {}
+    for n_fft in (0, -1):
+        step = HilbertStep(comp, n_imfs, offsets=offsets, row_base=row[:-1], dt=dt, n_fft=n_fft)
+        for _ in range(args.warmup):
+            step.run()
+        torch.cuda.synchronize()
+        st = torch.cuda.current_stream()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+        ev[0].record(st)
+        for i in range(args.steps):
+            step.run()
+            ev[i + 1].record(st)
+        torch.cuda.synchronize()
+        per = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
+        res[n_fft] = dict(ms=float(np.mean(per)), ms_min=float(np.min(per)), step=step)
+        _progress(f"hilbert n_fft={n_fft}: {res[n_fft]['ms']:.3f} ms per launch")
+    ms = res[0]["ms"]
+    alg_bytes, alg_flops = hilbert_work(lengths, nim, 0)
+    gbs = alg_bytes / (ms * 1e-3) / 1e9
+    tfs = alg_flops / (ms * 1e-3) / 1e12
+    # parity of the timed outputs with the oracle on a sample of traces
+    out = {k: v.cpu().numpy() for k, v in res[0]["step"].out.items()}
+    comp_h, rows_h = comp.cpu().numpy(), row.cpu().numpy()
+    worst = {"amp": 0.0, "phase": 0.0, "freq": 0.0}
+    sample = range(0, B, max(1, B // 256))
+    for s in sample:
+        for k in range(nim[s]):
+            a = rows_h[s] + k * N
+            r = comp_h[a:a + N]
+            ra, rp, rf = H.instantaneous(r, dt=dt)
+            well = ra >= 1e-3 * ra.max()
+            sc = np.max(np.abs(r))
+            worst["amp"] = max(worst["amp"], float(np.max(np.abs(out["amp"][a:a + N] - ra)) / sc))
+            dp = np.angle(np.exp(1j * (out["phase"][a:a + N] - rp)))
+            worst["phase"] = max(worst["phase"], float(np.max(np.abs(dp[well]), initial=0)))
+            m = np.minimum(np.arange(N), N - 2)
+            df = (out["freq"][a:a + N] - rf) * dt
+            df -= np.round(df)
+            worst["freq"] = max(worst["freq"], float(np.max(np.abs(df[well[m] & well[m + 1]]),
+                                                          initial=0)))
+    cpu = None
+    if pool is not None:
+        # the oracle (scipy.signal.hilbert + numpy) on the rows of a bounded sample of traces
+        pick = list(range(0, B, max(1, B // (64 * ncpu))))
+        rows = [comp_h[rows_h[s] + k * N: rows_h[s] + (k + 1) * N] for s in pick
+                for k in range(nim[s])]
+        chunks = [(rows[i::ncpu], dt) for i in range(ncpu)]
+        with pool:
+            pool.map(_hilbert_cpu_one, [(c[0][:2], dt) for c in chunks])
+            t0 = time.perf_counter()
+            done = sum(pool.map(_hilbert_cpu_one, chunks))
+            sec = time.perf_counter() - t0
+        cpu = {"value": done * N / sec, "unit": "samples/s", "cores": ncpu, "kind": "port",
+               "sample": f"{done} IMF rows of {len(pick)} traces, scipy.signal.hilbert + numpy "
+                         "angle/unwrap/diff, one process per core"}
+    samples = float(np.sum(nim)) * N
+    line = {
+        "metric": "Hilbert spectral step: IMF samples per second (amplitude, phase, frequency; "
+                  "fp64), seismic IMFs 16384 x N=2001, 1 GPU",
+        "value": samples / (ms * 1e-3), "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"IMFs of the seismic traces ({B} x N={N}, seed 1), compacted; "
+                               "scipy.signal.hilbert semantics (DFT of the row's length), "
+                               "dt = 4 ms", "traces": B, "N": N,
+                   "imf_rows": int(np.sum(nim))},
+        "roofline": {"bound": "valu" if tfs / FP64_PEAK_TFLOPS > gbs / HBM_PEAK_GBS else "hbm",
+                     "achieved_gbs": gbs, "peak_gbs": HBM_PEAK_GBS, "frac_hbm": gbs / HBM_PEAK_GBS,
+                     "achieved_tflops": tfs, "peak_tflops": FP64_PEAK_TFLOPS,
+                     "frac_fp64": tfs / FP64_PEAK_TFLOPS, "kernel_ms": ms,
+                     "algorithmic_bytes": alg_bytes, "algorithmic_flops": alg_flops,
+                     "traffic": measured_traffic("emd_hilbert_seismic_imfs_16384x2001.md")},
+        "traces_per_s": B / (ms * 1e-3),
+        "pow2_padded": {"n_fft": -1, "ms_per_step": res[-1]["ms"],
+                        "samples_per_s": samples / (res[-1]["ms"] * 1e-3)},
+        "parity_vs_oracle": {"traces": len(sample), "max_amp_err_rel": worst["amp"],
+                             "max_phase_err_rad": worst["phase"],
+                             "max_freq_err_cycles_per_sample": worst["freq"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -483,7 +624,8 @@ def main():
     ap.add_argument("--train", type=int, default=8192,
                     help="seismic traces (seed 3) the cost model is trained on before timing")
     ap.add_argument("--epochs", type=int, default=30)
-    ap.add_argument("--workload", choices=["seismic", "ragged", "costmodel", "stress", "segy"],
+    ap.add_argument("--workload", choices=["seismic", "ragged", "costmodel", "stress", "segy",
+                                           "hilbert"],
                     default="seismic",
                     help="seismic: the headline 16384 x N=2001 traces; ragged: the ragged mix "
                          "(N log-uniform in [256, 8192], seed 2; --batch sets B, default 65536); "
@@ -491,7 +633,8 @@ def main():
                          "on 262144 held-out traces (seed 4, 5%% white noise at N=8192); "
                          "stress: 262144 x N=4096 (seed 5) streamed from host in captured "
                          "chunks, max_extrema 2048, compacted output; segy: decode of a "
-                         "16384 x N=2001 IBM-float SEG-Y image in HBM")
+                         "16384 x N=2001 IBM-float SEG-Y image in HBM; hilbert: amplitude, "
+                         "phase and frequency of the seismic traces' compacted IMFs")
     ap.add_argument("--train-total", type=int, default=200000)
     ap.add_argument("--eval-total", type=int, default=262144)
     ap.add_argument("--tune-cache", default=None, help="autotuner JSON cache (ragged)")
@@ -519,6 +662,8 @@ def main():
         return bench_stress(args)
     if args.workload == "segy":
         return bench_segy(args)
+    if args.workload == "hilbert":
+        return bench_hilbert(args)
 
     from cuda_emd import synth
 

@@ -28,9 +28,15 @@ struct cpx {
 
 EMDH_HD cpx add(cpx a, cpx b) { return {a.re + b.re, a.im + b.im}; }
 EMDH_HD cpx sub(cpx a, cpx b) { return {a.re - b.re, a.im - b.im}; }
-EMDH_HD cpx mul(cpx a, cpx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+// Complex products with one fused multiply-add per part (correctly rounded on the host and
+// on the device alike).
+EMDH_HD cpx mul(cpx a, cpx b) {
+  return {fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re)};
+}
 // a * conj(b)
-EMDH_HD cpx mulc(cpx a, cpx b) { return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im}; }
+EMDH_HD cpx mulc(cpx a, cpx b) {
+  return {fma(a.re, b.re, a.im * b.im), fma(a.im, b.re, -(a.re * b.im))};
+}
 // a * (DIR i): DIR = -1 multiplies by -i, +1 by i
 template <int DIR>
 EMDH_HD cpx rot90(cpx a) {
@@ -66,7 +72,12 @@ EMDH_HD cpx rot16(cpx a, int m) {
   if (m == 8) return cpx{-a.re, -a.im};
   if (m == 12) return rot90<-DIR>(a);
   const double c = cos16(m), s = DIR * cos16(m - 4);
-  return cpx{a.re * c - a.im * s, a.re * s + a.im * c};
+  if ((m & 3) == 2) {
+    // |c| = |s| = sqrt(2)/2: (re c - im s, re s + im c) = c (re - (s/c) im, (s/c) re + im)
+    const double r = s / c;  // +-1, folded
+    return cpx{c * (a.re - r * a.im), c * (r * a.re + a.im)};
+  }
+  return cpx{fma(a.re, c, -(a.im * s)), fma(a.re, s, a.im * c)};
 }
 
 // In-place DFT of r = 2^LR points a[0..r): a[k] <- sum_n a[n] exp(DIR 2 pi i n k / r).
@@ -138,17 +149,103 @@ EMDH_HD void dft(cpx* a) {
   }
 }
 
+// sin and cos of pi x from two polynomials on the quarter r in [0, 1/4]:
+// sin(pi r) = r S(r^2), cos(pi r) = 1 + r^2 C(r^2), S then C in the table (highest degree
+// first, derived by emd-cuda_amd/csrc/poly_coefficients.py; errors below 1e-20 before
+// rounding).  Arguments are reduced to the quarter in integers, so the only rounding
+// before the polynomials is that of r itself.  The device kernel passes the table through
+// memory (scalar registers) for the same reason as atan2's.
+#define EMDH_TRIG_POLY                                                                       \
+  {-2.1717401400090533e-05, 0.00046628272951211076, -0.007370429884712669,                  \
+   0.08214588658005621,     -0.5992645293202978,    2.5501640398773415,                     \
+   -5.16771278004997,       3.141592653589793,                                              \
+   4.268505646755225e-06,   -0.0001046345989636836, 0.0019295741241061913,                  \
+   -0.025806891384587305,   0.23533063035880689,    -1.3352627688545888,                    \
+   4.0587121264167685,      -4.934802200544679}
+constexpr int kTrigTerms = 8;
+
+EMDH_HD void sincospi_quarter(double r, const double* T, double* s, double* c) {
+  const double u = r * r;
+  double p = T[0], q = T[kTrigTerms];
+#pragma unroll
+  for (int i = 1; i < kTrigTerms; ++i) {
+    p = fma(p, u, T[i]);
+    q = fma(q, u, T[kTrigTerms + i]);
+  }
+  *s = r * p;
+  *c = fma(u, q, 1.0);
+}
+
+// sin and cos of pi num / den for 0 <= num < 2 den: x >= 1 is x - 1 with both signs
+// flipped, x > 1/2 is 1 - x with the cosine's sign flipped, x > 1/4 is the cofunction of
+// 1/2 - x.  `lg_den` >= 0 when den = 2^lg_den (r = ldexp, exact), else r = num / den.
+EMDH_HD void sincospi_ratio(int num, int den, int lg_den, const double* T, double* s,
+                            double* c) {
+  double ss = 1.0, cs = 1.0;
+  if (num >= den) {
+    num -= den;
+    ss = -1.0;
+    cs = -1.0;
+  }
+  if (2 * num > den) {
+    num = den - num;
+    cs = -cs;
+  }
+  const bool co = 4 * num > den;
+  int rn = num, rd_lg = lg_den + 0;
+  int rd = den;
+  if (co) {  // r = 1/2 - num/den = (den - 2 num) / (2 den)
+    rn = den - 2 * num;
+    rd = 2 * den;
+    rd_lg = lg_den + 1;
+  }
+  const double r = lg_den >= 0 ? ldexp((double)rn, -rd_lg) : (double)rn / (double)rd;
+  double s0, c0;
+  sincospi_quarter(r, T, &s0, &c0);
+  *s = ss * (co ? c0 : s0);
+  *c = cs * (co ? s0 : c0);
+}
+
 // exp(-2 pi i q / 2^lg) for 0 <= q < 2^lg
-EMDH_HD cpx expm(int q, int lg) {
-  const double x = ldexp((double)q, 1 - lg);  // 2 q / 2^lg, exact
+EMDH_HD cpx expm(int q, int lg, const double* T) {
   double s, c;
-#if defined(__HIP_DEVICE_COMPILE__)
-  sincospi(x, &s, &c);
-#else
-  s = sin(M_PI * x);
-  c = cos(M_PI * x);
-#endif
+  sincospi_ratio(2 * q, 1 << lg, lg, T, &s, &c);  // angle pi (2q / 2^lg)
   return cpx{c, -s};
+}
+
+// atan2 for the phase outputs: t = min(|x|,|y|) / max(|x|,|y|) (one correctly rounded
+// division), atan(t) = t + t^3 P(t^2) on [0, 1] by Horner with fused multiply-adds, then
+// the quadrant: pi/2 - a, pi - a or pi/2 + a with pi/2 and pi as two-double sums.  Within
+// 1.5 ulp of the exact value; signed zeros as C's atan2 (atan2(+-0, -0) = +-pi).  P
+// (highest degree first) is derived by emd-cuda_amd/csrc/atan_coefficients.py.  The
+// coefficients are read from memory (the caller passes the table) so that the device
+// kernel keeps them in scalar registers instead of 42 vector registers.
+#define EMDH_ATAN_POLY                                                                     \
+  {-1.1832505417555692e-05, 0.0001368724853148122, -0.0007518472526973822,                \
+   0.002622977891906089,    -0.006575683344151699, 0.012756172907694298,                  \
+   -0.020238706986393514,   0.027567942297344678,  -0.033750132001619734,                 \
+   0.03872640214042632,     -0.04308119655330471,  0.04752086773576656,                   \
+   -0.05261265735709454,    0.05882074956371294,   -0.06666636435777692,                  \
+   0.07692305354678655,     -0.09090908969557403,  0.11111111107234799,                   \
+   -0.14285714285648404,    0.19999999999999554,   -0.3333333333333333}
+constexpr int kAtanTerms = 21;
+
+EMDH_HD double atan2_poly(double y, double x, const double* P) {
+  const double ax = fabs(x), ay = fabs(y);
+  const bool sw = ay > ax, ng = signbit(x);
+  const double mx = sw ? ay : ax, mn = sw ? ax : ay;
+  const double t = mx == 0.0 ? 0.0 : mn / mx;
+  const double s = t * t;
+  double p = P[0];
+#pragma unroll
+  for (int i = 1; i < kAtanTerms; ++i) p = fma(p, s, P[i]);
+  double a = fma(t * s, p, t);
+  constexpr double pio2_hi = 1.5707963267948966, pio2_lo = 6.123233995736766e-17;
+  constexpr double pi_hi = 3.141592653589793, pi_lo = 1.2246467991473532e-16;
+  if (sw && ng) a = (pio2_hi + a) + pio2_lo;
+  else if (sw) a = (pio2_hi - a) + pio2_lo;
+  else if (ng) a = (pi_hi - a) + pi_lo;
+  return copysign(a, y);
 }
 
 // Compile-time plan of an F = 2^LOGF point transform: stages of radix 16, the last one of
@@ -211,12 +308,18 @@ struct Plan {
     return sg * (1.0 / F);
   }
   EMDH_HD static int pad(int p) { return p + (p >> 4); }
+  // LDS index of register e of lane t in stage s, as the lane's base plus a per-register
+  // constant: pad(pos(s, t, e)) - pad(pos(s, t, 0)) does not depend on t (a butterfly's
+  // points never straddle a pad boundary differently from lane to lane), so the offsets
+  // fold into the LDS instructions' immediates.
+  EMDH_HD static int lds_base(int s, int t) { return pad(pos(s, t, 0)); }
+  EMDH_HD static int lds_off(int s, int e) { return pad(pos(s, 0, e)) - pad(pos(s, 0, 0)); }
 };
 
 // Forward (DIF) stage s on one lane's registers: for each butterfly, the length-r DFT,
 // then output k times exp(-2 pi i q k / L).
 template <int LOGF, int S>
-EMDH_HD void stage_fwd(cpx* a, int t) {
+EMDH_HD void stage_fwd(cpx* a, int t, const double* T) {
   using P = Plan<LOGF>;
   constexpr int lr = P::log_radix(S), r = 1 << lr, lL = P::log_block(S);
 #pragma unroll
@@ -224,7 +327,7 @@ EMDH_HD void stage_fwd(cpx* a, int t) {
     cpx* v = a + i * r;
     dft<lr, -1>(v);
     if constexpr (lL > lr) {
-      const cpx w = expm(P::qoff(S, t, i * r), lL);
+      const cpx w = expm(P::qoff(S, t, i * r), lL, T);
       cpx wk = w;
 #pragma unroll
       for (int k = 1; k < r; ++k) {
@@ -238,14 +341,14 @@ EMDH_HD void stage_fwd(cpx* a, int t) {
 // Inverse of stage_fwd (without the 1/r): output k times exp(+2 pi i q k / L), then the
 // length-r inverse DFT.
 template <int LOGF, int S>
-EMDH_HD void stage_inv(cpx* a, int t) {
+EMDH_HD void stage_inv(cpx* a, int t, const double* T) {
   using P = Plan<LOGF>;
   constexpr int lr = P::log_radix(S), r = 1 << lr, lL = P::log_block(S);
 #pragma unroll
   for (int i = 0; i < P::R / r; ++i) {
     cpx* v = a + i * r;
     if constexpr (lL > lr) {
-      const cpx w = expm(P::qoff(S, t, i * r), lL);
+      const cpx w = expm(P::qoff(S, t, i * r), lL, T);
       cpx wk = w;
 #pragma unroll
       for (int k = 1; k < r; ++k) {
@@ -264,36 +367,23 @@ EMDH_HD void stage_inv(cpx* a, int t) {
 //   N even: 2 cot(pi n / N) / N for odd n, 0 for even n.
 // (The half-angle forms of (cos(pi n/N) - (-1)^n) / (N sin(pi n/N)), free of the
 // cancellation of the latter at small n.)
-EMDH_HD double hilbert_kernel_value(int n, int N) {
+EMDH_HD double hilbert_kernel_value(int n, int N, const double* T) {
+  double s, c;
   if (N & 1) {
-    double s, c;
-    const double x = (double)n / (2.0 * N);
-#if defined(__HIP_DEVICE_COMPILE__)
-    sincospi(x, &s, &c);
-#else
-    s = sin(M_PI * x);
-    c = cos(M_PI * x);
-#endif
+    sincospi_ratio(n, 2 * N, -1, T, &s, &c);  // angle pi n / 2N
     return (n & 1) ? c / (s * N) : -s / (c * N);
   }
   if (!(n & 1)) return 0.0;
-  double s, c;
-  const double x = (double)n / N;
-#if defined(__HIP_DEVICE_COMPILE__)
-  sincospi(x, &s, &c);
-#else
-  s = sin(M_PI * x);
-  c = cos(M_PI * x);
-#endif
+  sincospi_ratio(n, N, -1, T, &s, &c);  // angle pi n / N
   return 2.0 * c / (s * N);
 }
 
 // The kernel laid out circularly on F >= 2N-1 points, so that a length-F circular
 // convolution reproduces the length-N one on its first N outputs: g[p] = q[p] for
 // 1 <= p <= N-1, g[p] = -q[F-p] for F-N+1 <= p <= F-1 (the kernel is odd), else 0.
-EMDH_HD double circular_kernel(int p, int N, int F) {
-  if (p >= 1 && p <= N - 1) return hilbert_kernel_value(p, N);
-  if (p >= F - N + 1 && p <= F - 1) return -hilbert_kernel_value(F - p, N);
+EMDH_HD double circular_kernel(int p, int N, int F, const double* T) {
+  if (p >= 1 && p <= N - 1) return hilbert_kernel_value(p, N, T);
+  if (p >= F - N + 1 && p <= F - 1) return -hilbert_kernel_value(F - p, N, T);
   return 0.0;
 }
 

@@ -9,11 +9,12 @@
 // kernel on 2^ceil(log2(2N-1)) points; for a power-of-two length it multiplies by
 // -i sgn(k) directly (emd_hilbert.h).
 //
-// One workgroup per trace (grid-stride): the trace's rows are transformed two at a time
-// (x1 + i x2) with the points in registers and the stage exchanges in LDS, then each row's
-// outputs are written in natural order, coalesced.  Every launch is one transform class
-// (one length and form); a batch of mixed lengths takes one launch per class, each
-// skipping the traces of other classes.
+// One workgroup per trace: the trace's rows are transformed two at a time (x1 + i x2) with
+// the points in registers and the stage exchanges in LDS, then each row's outputs are
+// written in natural order, coalesced.  Every launch is one transform class (one length and
+// form): a batch whose rows share one class is one launch of a workgroup per trace; a batch
+// of mixed lengths takes one launch per class its length range spans, each walking the
+// batch with the workgroups resident at once and skipping the traces of other classes.
 #include "../../include/emd_gfx950.h"
 
 #include <hip/hip_runtime.h>
@@ -59,23 +60,40 @@ __device__ __forceinline__ void trace_geometry(const Rows& a, int s, int& len, i
   base = a.row_base ? a.row_base[s] : off * a.max_imfs;
 }
 
-// The outputs of sample n of a row with analytic signal x + i h; h of the whole row is in
-// hrow (LDS), the row's samples in x (global).
-__device__ __forceinline__ void emit_sample(const Rows& a, const double* x, const double* hrow,
-                                            int64_t out, int n, int len, double xv, double h) {
+__constant__ double kAtanPoly[kAtanTerms] = EMDH_ATAN_POLY;
+__constant__ double kTrigPoly[2 * kTrigTerms] = EMDH_TRIG_POLY;
+
+// A polynomial table's address behind an empty asm: the table's loads stay where it is
+// used (scalar loads into scalar registers) instead of its constants being hoisted out of
+// every loop into vector registers held through the whole kernel.
+__device__ __forceinline__ const double* table(const double* p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+// Amplitude and phase of sample n of a row with analytic signal x + i h; returns the phase.
+__device__ __forceinline__ double emit_amp_phase(const Rows& a, int64_t out, int n, double xv,
+                                                 double h) {
   if (a.amp) a.amp[out + n] = sqrt(xv * xv + h * h);
-  if (a.phase) a.phase[out + n] = atan2(h, xv);
-  if (a.freq) {
-    double f = 0.0;
-    if (len > 1) {
-      // the phase increment from m to m+1, with m = n except at the last sample
-      const int m = n < len - 1 ? n : len - 2;
-      const double x0 = m == n ? xv : x[m], h0 = m == n ? h : hrow[m];
-      const double x1 = x[m + 1], h1 = hrow[m + 1];
-      f = atan2(x0 * h1 - h0 * x1, x0 * x1 + h0 * h1) * a.inv_2pi_dt;
-    }
-    a.freq[out + n] = f;
+  const double ph = atan2_poly(h, xv, table(kAtanPoly));
+  if (a.phase) a.phase[out + n] = ph;
+  return ph;
+}
+
+// Frequency of sample n from the row's phases: the increment from m to m+1 (m = n except at
+// the last sample) wrapped into [-pi, pi] as numpy.unwrap does (an increment of exactly
+// +-pi is kept).
+__device__ __forceinline__ void emit_freq(const Rows& a, const double* ph, int64_t out, int n,
+                                          int len) {
+  double f = 0.0;
+  if (len > 1) {
+    const int m = n < len - 1 ? n : len - 2;
+    double d = ph[m + 1] - ph[m];
+    if (d > M_PI) d -= 2.0 * M_PI;
+    else if (d < -M_PI) d += 2.0 * M_PI;
+    f = d * a.inv_2pi_dt;
   }
+  a.freq[out + n] = f;
 }
 
 // Points move from their stage-FROM positions to their stage-TO positions through LDS,
@@ -84,25 +102,27 @@ template <int LOGF, int FROM, int TO>
 __device__ __forceinline__ void exchange(double* xch, cpx* v, bool act) {
   using P = Plan<LOGF>;
   const int t = fresh_lane();
+  double* wb = xch + P::lds_base(FROM, t);
+  double* rb = xch + P::lds_base(TO, t);
   __syncthreads();
   if (act) {
 #pragma unroll
-    for (int e = 0; e < P::R; ++e) xch[P::pad(P::pos(FROM, t, e))] = v[e].re;
+    for (int e = 0; e < P::R; ++e) wb[P::lds_off(FROM, e)] = v[e].re;
   }
   __syncthreads();
   if (act) {
 #pragma unroll
-    for (int e = 0; e < P::R; ++e) v[e].re = xch[P::pad(P::pos(TO, t, e))];
+    for (int e = 0; e < P::R; ++e) v[e].re = rb[P::lds_off(TO, e)];
   }
   __syncthreads();
   if (act) {
 #pragma unroll
-    for (int e = 0; e < P::R; ++e) xch[P::pad(P::pos(FROM, t, e))] = v[e].im;
+    for (int e = 0; e < P::R; ++e) wb[P::lds_off(FROM, e)] = v[e].im;
   }
   __syncthreads();
   if (act) {
 #pragma unroll
-    for (int e = 0; e < P::R; ++e) v[e].im = xch[P::pad(P::pos(TO, t, e))];
+    for (int e = 0; e < P::R; ++e) v[e].im = rb[P::lds_off(TO, e)];
   }
 }
 
@@ -110,7 +130,7 @@ template <int LOGF, int S>
 __device__ __forceinline__ void forward(double* xch, cpx* v, bool act) {
   if constexpr (S < Plan<LOGF>::NS) {
     if constexpr (S > 0) exchange<LOGF, S - 1, S>(xch, v, act);
-    if (act) stage_fwd<LOGF, S>(v, fresh_lane());
+    if (act) stage_fwd<LOGF, S>(v, fresh_lane(), table(kTrigPoly));
     forward<LOGF, S + 1>(xch, v, act);
   }
 }
@@ -118,19 +138,20 @@ __device__ __forceinline__ void forward(double* xch, cpx* v, bool act) {
 template <int LOGF, int S>
 __device__ __forceinline__ void inverse(double* xch, cpx* v, bool act) {
   if constexpr (S >= 0) {
-    if (act) stage_inv<LOGF, S>(v, fresh_lane());
+    if (act) stage_inv<LOGF, S>(v, fresh_lane(), table(kTrigPoly));
     if constexpr (S > 0) exchange<LOGF, S, S - 1>(xch, v, act);
     inverse<LOGF, S - 1>(xch, v, act);
   }
 }
 
 // Writes the outputs of one row whose H x sits in v[e].re (IM false) or v[e].im (IM true)
-// at the stage-0 positions of the lane.
+// at the stage-0 positions of the lane: H x goes to LDS, each sample's phase replaces it
+// there, and the frequencies are the differences of neighbouring phases.
 template <int LOGF, bool IM>
 __device__ __forceinline__ void emit_row(const Rows& a, double* xch, const double* x,
                                          int64_t out, const cpx* v, int len, bool act) {
   using P = Plan<LOGF>;
-  int t = fresh_lane();
+  const int t = fresh_lane();
   __syncthreads();
   if (act) {
 #pragma unroll
@@ -143,7 +164,46 @@ __device__ __forceinline__ void emit_row(const Rows& a, double* xch, const doubl
   // samples in natural order, a workgroup-wide stride at a time (coalesced, and a rolled
   // loop: the transcendental outputs are the bulk of the code)
 #pragma unroll 1
-  for (int n = threadIdx.x; n < len; n += P::WG) emit_sample(a, x, xch, out, n, len, x[n], xch[n]);
+  for (int n = threadIdx.x; n < len; n += P::WG) xch[n] = emit_amp_phase(a, out, n, x[n], xch[n]);
+  if (a.freq) {
+    __syncthreads();
+#pragma unroll 1
+    for (int n = threadIdx.x; n < len; n += P::WG) emit_freq(a, xch, out, n, len);
+  }
+}
+
+// Both rows of a pair at once: H x1 at xch[0, len), H x2 at xch[len, 2 len) (2 len <= F + 1
+// in the convolution form), so no transform register is live through the output loops.
+template <int LOGF>
+__device__ __forceinline__ void emit_pair(const Rows& a, double* xch, const double* x1,
+                                          int64_t out, const cpx* v, int len, bool two,
+                                          bool act) {
+  using P = Plan<LOGF>;
+  const int t = fresh_lane();
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int e = 0; e < P::R; ++e) {
+      const int n = P::pos(0, t, e);
+      if (n < len) {
+        xch[n] = v[e].re;
+        xch[len + n] = v[e].im;
+      }
+    }
+  }
+  __syncthreads();
+  // the two rows are adjacent in memory (x2 = x1 + len), as are their outputs
+  const int cnt = two ? 2 * len : len;
+#pragma unroll 1
+  for (int n = threadIdx.x; n < cnt; n += P::WG) xch[n] = emit_amp_phase(a, out, n, x1[n], xch[n]);
+  if (a.freq) {
+    __syncthreads();
+#pragma unroll 1
+    for (int n = threadIdx.x; n < cnt; n += P::WG) {
+      const int r = n >= len;
+      emit_freq(a, xch + r * len, out + r * len, n - r * len, len);
+    }
+  }
 }
 
 template <int LOGF, bool EXACT>
@@ -166,8 +226,9 @@ __global__ __launch_bounds__(Plan<LOGF>::WG) void hilbert_kernel(Rows a) {
     if constexpr (EXACT) {
       cpx g[P::R];
       const int t = fresh_lane();
+      const double* T = table(kTrigPoly);
 #pragma unroll
-      for (int e = 0; e < P::R; ++e) g[e] = cpx{circular_kernel(P::pos(0, t, e), len, P::F), 0.0};
+      for (int e = 0; e < P::R; ++e) g[e] = cpx{circular_kernel(P::pos(0, t, e), len, P::F, T), 0.0};
       forward<LOGF, 0>(xch, g, act);
 #pragma unroll
       for (int e = 0; e < P::R; ++e) S[e] = -g[e].im * (1.0 / P::F);
@@ -199,8 +260,12 @@ __global__ __launch_bounds__(Plan<LOGF>::WG) void hilbert_kernel(Rows a) {
         }
       }
       inverse<LOGF, P::NS - 1>(xch, v, act);
-      emit_row<LOGF, false>(a, xch, x1, out, v, len, act);
-      if (two) emit_row<LOGF, true>(a, xch, x2, out + len, v, len, act);
+      if constexpr (EXACT) {
+        emit_pair<LOGF>(a, xch, x1, out, v, len, two, act);
+      } else {
+        emit_row<LOGF, false>(a, xch, x1, out, v, len, act);
+        if (two) emit_row<LOGF, true>(a, xch, x2, out + len, v, len, act);
+      }
     }
   }
 }
@@ -214,11 +279,13 @@ __global__ __launch_bounds__(256) void hilbert_trivial_kernel(Rows a) {
     bool exact;
     const int nim = a.n_imfs[s];
     if (nim <= 0 || len <= 0 || transform_class(len, a.n_fft, &exact) != kTrivial) continue;
-    const double zero[2] = {0.0, 0.0};
     for (int r = 0; r < nim; ++r) {
       const int64_t out = base + (int64_t)r * len;
       const double* x = a.imfs + out;
-      for (int n = 0; n < len; ++n) emit_sample(a, x, zero, out, n, len, x[n], 0.0);
+      double ph[2];
+      for (int n = 0; n < len; ++n) ph[n] = emit_amp_phase(a, out, n, x[n], 0.0);
+      if (a.freq)
+        for (int n = 0; n < len; ++n) emit_freq(a, ph, out, n, len);
     }
   }
 }
@@ -324,12 +391,13 @@ bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 extern "C" {
 
 int emd_hilbert(const double* imfs, const int64_t* row_base, int max_imfs, const int* n_imfs,
-                const int64_t* offsets, int N, int B, int max_n, int n_fft, double dt,
+                const int64_t* offsets, int N, int B, int min_n, int max_n, int n_fft, double dt,
                 double* amp, double* phase, double* freq, void* stream) {
   using namespace emdh;
   if (B < 0) return emdk::fail(EMD_ERR_VALUE, "batch size must be >= 0, got %d", B);
   if (!offsets && N < 0) return emdk::fail(EMD_ERR_VALUE, "N must be >= 0, got %d", N);
-  if (offsets && max_n < 0) return emdk::fail(EMD_ERR_VALUE, "max_n must be >= 0, got %d", max_n);
+  if (offsets && (min_n < 0 || max_n < min_n))
+    return emdk::fail(EMD_ERR_VALUE, "need 0 <= min_n <= max_n, got %d and %d", min_n, max_n);
   if (!row_base && max_imfs < 1)
     return emdk::fail(EMD_ERR_VALUE, "the default row layout needs max_imfs >= 1");
   if (!(dt > 0.0) || !std::isfinite(dt))
@@ -346,7 +414,8 @@ int emd_hilbert(const double* imfs, const int64_t* row_base, int max_imfs, const
     return emdk::fail(EMD_ERR_VALUE, "transform length above %d", 1 << kMaxLogF);
   if (n_fft == 0) {
     bool ex;
-    const bool ok = offsets ? longest <= (1 << (kMaxLogF - 1))
+    const bool ok = offsets ? (longest <= (1 << (kMaxLogF - 1)) ||
+                               (min_n == max_n && transform_class(longest, 0, &ex) != 0))
                             : (longest == 0 || transform_class(longest, 0, &ex) != 0);
     if (!ok)
       return emdk::fail(EMD_ERR_VALUE,
@@ -359,31 +428,31 @@ int emd_hilbert(const double* imfs, const int64_t* row_base, int max_imfs, const
   Rows a{imfs, row_base, max_imfs, n_imfs, offsets, N, B, n_fft,
          1.0 / (2.0 * M_PI * dt), amp, phase, freq};
   const hipStream_t st = (hipStream_t)stream;
-  int rc;
-  if (!offsets) {
-    bool exact;
-    const int c = transform_class(N, n_fft, &exact);
-    return c == kTrivial ? launch_trivial(a, st) : launch_class(a, c, exact, true, st);
-  }
-  // every class a length in [1, max_n] can fall in
   bool exact;
-  if (n_fft > 0) {
-    const int c = transform_class(1, n_fft, &exact);
-    return c == kTrivial ? launch_trivial(a, st) : launch_class(a, c, false, true, st);
+  if (!offsets || n_fft > 0) {
+    // one transform class for every row
+    const int c = transform_class(offsets ? 1 : N, n_fft, &exact);
+    if (c == kTrivial) return launch_trivial(a, st);
+    return launch_class(a, c, exact, true, st);
   }
-  if ((rc = launch_trivial(a, st))) return rc;
+  // every class a length in [min_n, max_n] can fall in
+  const int lo = min_n < 1 ? 1 : min_n;
+  int rc;
+  if (lo <= 2 && (rc = launch_trivial(a, st))) return rc;
   if (max_n < 3) return EMD_OK;
-  const int top_pow2 = transform_class(max_n, -1, &exact);  // ceil(log2(max_n))
-  for (int c = kMinLogF; c <= top_pow2; ++c) {
-    // n_fft = -1: every length in (2^(c-1), 2^c]; n_fft = 0: only the length 2^c
-    if (n_fft == 0 && (1 << c) > max_n) break;
+  const int lo3 = lo < 3 ? 3 : lo;
+  if (n_fft < 0) {
+    for (int c = ceil_log2(lo3); c <= ceil_log2(max_n); ++c)
+      if ((rc = launch_class(a, c, false, false, st))) return rc;
+    return EMD_OK;
+  }
+  // the DFT of the row's own length: power-of-two lengths directly ...
+  for (int c = ceil_log2(lo3 < 4 ? 4 : lo3); (1 << c) <= max_n; ++c)
     if ((rc = launch_class(a, c, false, false, st))) return rc;
-  }
-  if (n_fft == 0) {
-    const int top_exact = ceil_log2(2 * max_n - 1);
-    for (int c = 3; c <= top_exact; ++c)
+  // ... the others as convolutions on 2^ceil(log2(2 len - 1)) points
+  if (min_n != max_n || (max_n & (max_n - 1)) != 0)
+    for (int c = ceil_log2(2 * lo3 - 1); c <= ceil_log2(2 * max_n - 1); ++c)
       if ((rc = launch_class(a, c, true, false, st))) return rc;
-  }
   return EMD_OK;
 }
 

@@ -84,8 +84,8 @@ PROTOTYPES = {
     "emd_segy_decode": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int, _vp, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, _vp, _vp]),
     "emd_hilbert": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int,
-                                   ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _vp, _vp,
-                                   _vp]),
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                   _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

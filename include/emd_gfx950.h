@@ -243,18 +243,21 @@ int emd_segy_decode(const void* bytes, const int64_t* trace_pos, int64_t trace_s
  *   n_fft  > 0:  the same with L = n_fft for every row (a power of two >= max_n, <= 16384).
  * amp[n] = |z[n]|, phase[n] = atan2(Im z[n], Re z[n]) (numpy.angle), and
  * freq[n] = (phi[n+1] - phi[n]) / (2 pi dt) with phi = numpy.unwrap(numpy.angle(z)) for
- * n < len-1 (the increment wrapped into [-pi, pi]), freq[len-1] = freq[len-2], and 0 for a
- * one-sample row; dt is the sample interval (freq in cycles per unit of dt).
+ * n < len-1 (the phase increment wrapped into [-pi, pi], +-pi kept), freq[len-1] =
+ * freq[len-2], and 0 for a one-sample row; dt is the sample interval (freq in cycles per
+ * unit of dt).
  * Rows: trace s has n_imfs[s] rows of len(s) samples (offsets[s+1] - offsets[s], or N when
  * offsets is NULL) back to back from element row_base[s] of imfs; row_base NULL is the
  * layout of emd_run_padded (s*max_imfs*N) or emd_run_ragged (offsets[s]*max_imfs), and
  * emd_compact's row_offsets is a valid row_base.  amp, phase and freq (each may be NULL)
- * share imfs' layout; no other element is written.  With offsets, max_n bounds every
- * length.  Asynchronous on `stream`; allocates nothing.
+ * share imfs' layout; no other element is written.  With offsets, every length lies in
+ * [min_n, max_n] (rows outside it are skipped); one launch per transform length the range
+ * spans, one in all when every row has the same length.  Asynchronous on `stream`;
+ * allocates nothing.
  * No reference counterpart: the reference stops at the IMFs of traces it describes as
  * seismic, sampled at 4 ms (emd.py:28-30); this is the usual next step on them. */
 int emd_hilbert(const double* imfs, const int64_t* row_base, int max_imfs, const int* n_imfs,
-                const int64_t* offsets, int N, int B, int max_n, int n_fft, double dt,
+                const int64_t* offsets, int N, int B, int min_n, int max_n, int n_fft, double dt,
                 double* amp, double* phase, double* freq, void* stream);
 
 #ifdef __cplusplus

@@ -12,6 +12,9 @@ using namespace emdh;
 
 namespace {
 
+const double kTrig[2 * kTrigTerms] = EMDH_TRIG_POLY;
+const double kAtan[kAtanTerms] = EMDH_ATAN_POLY;
+
 template <int LOGF>
 struct Emu {
   using P = Plan<LOGF>;
@@ -28,10 +31,10 @@ struct Emu {
       for (double& d : lds) d = NAN;
       for (int t = 0; t < P::T; ++t)
         for (int e = 0; e < P::R; ++e)
-          lds[P::pad(P::pos(FROM, t, e))] = part ? lane(t)[e].im : lane(t)[e].re;
+          lds[P::lds_base(FROM, t) + P::lds_off(FROM, e)] = part ? lane(t)[e].im : lane(t)[e].re;
       for (int t = 0; t < P::T; ++t)
         for (int e = 0; e < P::R; ++e) {
-          const double d = lds[P::pad(P::pos(TO, t, e))];
+          const double d = lds[P::lds_base(TO, t) + P::lds_off(TO, e)];
           (part ? lane(t)[e].im : lane(t)[e].re) = d;
         }
     }
@@ -40,14 +43,14 @@ struct Emu {
   void forward() {
     if constexpr (S < P::NS) {
       if constexpr (S > 0) exchange<S - 1, S>();
-      for (int t = 0; t < P::T; ++t) stage_fwd<LOGF, S>(lane(t), t);
+      for (int t = 0; t < P::T; ++t) stage_fwd<LOGF, S>(lane(t), t, kTrig);
       forward<S + 1>();
     }
   }
   template <int S>
   void inverse() {
     if constexpr (S >= 0) {
-      for (int t = 0; t < P::T; ++t) stage_inv<LOGF, S>(lane(t), t);
+      for (int t = 0; t < P::T; ++t) stage_inv<LOGF, S>(lane(t), t, kTrig);
       if constexpr (S > 0) exchange<S, S - 1>();
       inverse<S - 1>();
     }
@@ -59,7 +62,7 @@ struct Emu {
     if (exact) {
       for (int t = 0; t < P::T; ++t)
         for (int e = 0; e < P::R; ++e)
-          lane(t)[e] = cpx{circular_kernel(P::pos(0, t, e), len, P::F), 0.0};
+          lane(t)[e] = cpx{circular_kernel(P::pos(0, t, e), len, P::F, kTrig), 0.0};
       forward<0>();
       for (int t = 0; t < P::T; ++t)
         for (int e = 0; e < P::R; ++e) S[t * P::R + e] = -lane(t)[e].im * (1.0 / P::F);
@@ -95,8 +98,9 @@ struct Emu {
   }
 
   // 0 when every stage's (lane, register) -> position map is a bijection onto [0, F)
-  // whose padded LDS index stays inside the exchange buffer, the twiddle offsets are the
-  // butterflies' q, and the digit reversal is a bijection.
+  // whose padded LDS index stays inside the exchange buffer and equals the lane's base plus
+  // the register's offset, the twiddle offsets are the butterflies' q, and the digit
+  // reversal is a bijection.
   static int check_maps() {
     for (int s = 0; s < P::NS; ++s) {
       std::vector<int> seen(P::F, 0);
@@ -105,6 +109,7 @@ struct Emu {
           const int p = P::pos(s, t, e);
           if (p < 0 || p >= P::F) return 1;
           if (P::pad(p) < 0 || P::pad(p) >= P::XCH) return 2;
+          if (P::lds_base(s, t) + P::lds_off(s, e) != P::pad(p)) return 7;
           ++seen[p];
           const int lr = P::log_radix(s), lLr = P::log_block(s) - lr;
           if (P::qoff(s, t, e) != (p & ((1 << lLr) - 1))) return 3;
@@ -163,6 +168,15 @@ int hilbert_emu(const double* x, int nrows, int len, int n_fft, double* h) {
   }
   return c;
 }
+
+// The kernel's own atan2 and exp(-2 pi i q / 2^lg), for accuracy checks.
+double hilbert_emu_atan2(double y, double x) { return atan2_poly(y, x, kAtan); }
+void hilbert_emu_expm(int q, int lg, double* out) {
+  const cpx w = expm(q, lg, kTrig);
+  out[0] = w.re;
+  out[1] = w.im;
+}
+double hilbert_emu_kernel_value(int n, int N) { return hilbert_kernel_value(n, N, kTrig); }
 
 int hilbert_emu_class(int len, int n_fft, int* exact) {
   bool ex;
