@@ -33,6 +33,15 @@
 
 #include "emd_sift.h"
 
+// Arrays carved out of a workgroup's LDS or global region: plain pointers, or bounds-checked
+// spans in a host emulation build (EMD_BOUNDS_CHECK, emd_bounds.h).
+#ifdef EMD_BOUNDS_CHECK
+#include "emd_bounds.h"
+#else
+#define EMD_ARR(T) T*
+#define EMD_SPAN(T, ptr, count, name) (ptr)
+#endif
+
 namespace emdk {
 
 constexpr double kEnergyRatioThr = 0.2;
@@ -48,23 +57,24 @@ constexpr int kWorkBytesPerBlock = 568;  // 64 fp64 + 4 fp64 + 2 u64 + 2 int per
 enum SplineKind { kConst = 0, kLinear = 1, kThreePoint = 2, kNotAKnot = 3 };
 
 struct Tile {
-  double* Y;   // knot values
-  double* M;   // chord slope of the interval that starts at each knot slot, overwritten
-               // by the solve with the second derivative at the slot
-  double* H;   // 1/h of the interval that starts at each knot slot
-  int* X;      // knot positions (sample index; mirrored knots may be <0 or >N-1)
-  double* R;   // solver rows of systems longer than 64 * (kRegRows + 2) rows, in the
-               // workgroup's global region: side s, array a (g, u, v, w) at
-               // R + (4 * s + a) * rows; null when the launch never needs them
+  EMD_ARR(double) Y;  // knot values
+  EMD_ARR(double) M;  // chord slope of the interval that starts at each knot slot,
+                      // overwritten by the solve with the second derivative at the slot
+  EMD_ARR(double) H;  // 1/h of the interval that starts at each knot slot
+  EMD_ARR(int) X;     // knot positions (sample index; mirrored knots may be <0 or >N-1)
+  EMD_ARR(double) R;  // solver rows of systems longer than 64 * (kRegRows + 2) rows, in the
+                      // workgroup's global region: side s, array a (g, u, v, w) at
+                      // R + (4 * s + a) * rows; null when the launch never needs them
   int rows;
 };
 
 __device__ __forceinline__ Tile make_tile(double* base, int slots) {
   Tile t;
-  t.Y = base;
-  t.M = base + slots;
-  t.H = base + 2 * slots;
-  t.X = reinterpret_cast<int*>(base + 3 * slots);
+  t.Y = EMD_SPAN(double, base, slots, "tile.Y");
+  t.M = EMD_SPAN(double, base + slots, slots, "tile.M");
+  t.H = EMD_SPAN(double, base + 2 * slots, slots, "tile.H");
+  int* x = reinterpret_cast<int*>(base + 3 * slots);
+  t.X = EMD_SPAN(int, x, slots, "tile.X");
   t.R = nullptr;
   t.rows = 0;
   return t;
@@ -91,24 +101,27 @@ struct SiftShared {
 // Per-trace LDS work area (ahead of the knot tile): the IMF and per-64-sample-block
 // bookkeeping, laid out for the launch's longest trace.
 struct Work {
-  double* imf;      // [nblk*64] current IMF
-  double* bsum;     // [4][nblk] block partial sums: the sift's three sums (rows 0-2), the
-                    // accept pass's |residual| (row 0) and an IMF start's sum of squares (row 3)
-  uint64_t* mmax;   // [nblk] maxima masks of the last detection
-  uint64_t* mmin;   // [nblk] minima masks
-  int* pmax;        // [nblk] exclusive prefix counts of maxima over blocks
-  int* pmin;        // [nblk]
-  int nbcap;        // blocks of the layout
+  EMD_ARR(double) imf;     // [nblk*64] current IMF
+  EMD_ARR(double) bsum;    // [4][nblk] block partial sums: the sift's three sums (rows 0-2),
+                           // the accept pass's |residual| (row 0) and an IMF start's sum of
+                           // squares (row 3)
+  EMD_ARR(uint64_t) mmax;  // [nblk] maxima masks of the last detection
+  EMD_ARR(uint64_t) mmin;  // [nblk] minima masks
+  EMD_ARR(int) pmax;       // [nblk] exclusive prefix counts of maxima over blocks
+  EMD_ARR(int) pmin;       // [nblk]
+  int nbcap;               // blocks of the layout
 };
 
 __device__ __forceinline__ Work make_work(double* base, int nbcap) {
   Work w;
-  w.imf = base;
-  w.bsum = base + 64 * nbcap;
-  w.mmax = reinterpret_cast<uint64_t*>(base + 68 * nbcap);
-  w.mmin = w.mmax + nbcap;
-  w.pmax = reinterpret_cast<int*>(w.mmin + nbcap);
-  w.pmin = w.pmax + nbcap;
+  w.imf = EMD_SPAN(double, base, 64 * nbcap, "work.imf");
+  w.bsum = EMD_SPAN(double, base + 64 * nbcap, 4 * nbcap, "work.bsum");
+  uint64_t* masks = reinterpret_cast<uint64_t*>(base + 68 * nbcap);
+  w.mmax = EMD_SPAN(uint64_t, masks, nbcap, "work.mmax");
+  w.mmin = EMD_SPAN(uint64_t, masks + nbcap, nbcap, "work.mmin");
+  int* prefix = reinterpret_cast<int*>(masks + 2 * nbcap);
+  w.pmax = EMD_SPAN(int, prefix, nbcap, "work.pmax");
+  w.pmin = EMD_SPAN(int, prefix + nbcap, nbcap, "work.pmin");
   w.nbcap = nbcap;
   return w;
 }
@@ -380,17 +393,17 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
   const int tid = fresh_tid(), lane = tid & 63, w = uni(tid >> 6);
   const int pad = P.nbsym;
   const int nblk = ceil_div(N, 64);
-  const double* imf = wk.imf;
+  const EMD_ARR(double) imf = wk.imf;
 
   // ---- pass A: per-block lane masks of maxima/minima, zero crossings, plateaus --------
   int nz = 0, plat = 0;
   for (int blk = w; blk < nblk; blk += NW) {
     const int i = blk * 64 + lane;
-    // all three loads stay inside the work area (i + 1 <= 64 * nbcap lands in the block
-    // sums that follow the IMF); values outside the trace are masked off below
+    // all three loads stay inside the IMF array (the neighbours are clamped at its ends);
+    // values outside the trace are masked off below
     const double x = imf[i];
     const double prev = imf[i >= 1 ? i - 1 : 0];
-    const double next = imf[i + 1];
+    const double next = imf[min(i + 1, 64 * wk.nbcap - 1)];
     const double d2 = next - x;
     const double d1 = x - prev;
     // lanes whose sample is in the trace (i < N), has a successor (i <= N - 2) and, for
@@ -837,10 +850,10 @@ __device__ __forceinline__ void spline_system_wave(const Tile& t, int side, int 
   const int e = act ? s + base + (lane < rem ? 1 : 0) - 1 : 0;
   const int len = act ? e - s - 1 : 0;  // interior rows of this chunk
   double rg[kRegRows], ru[kRegRows], rv[kRegRows], rw[kRegRows];
-  double* Rg = t.R ? t.R + (4 * side + 0) * t.rows : nullptr;
-  double* Ru = t.R ? t.R + (4 * side + 1) * t.rows : nullptr;
-  double* Rv = t.R ? t.R + (4 * side + 2) * t.rows : nullptr;
-  double* Rw = t.R ? t.R + (4 * side + 3) * t.rows : nullptr;
+  EMD_ARR(double) Rg = t.R ? t.R + (4 * side + 0) * t.rows : nullptr;
+  EMD_ARR(double) Ru = t.R ? t.R + (4 * side + 1) * t.rows : nullptr;
+  EMD_ARR(double) Rv = t.R ? t.R + (4 * side + 2) * t.rows : nullptr;
+  EMD_ARR(double) Rw = t.R ? t.R + (4 * side + 3) * t.rows : nullptr;
   // chunk end rows after eliminating the interior (identity rows on idle lanes)
   double rsa = 0.0, rsb = 1.0, rsc = 0.0, rsd = 0.0;
   double rea = 0.0, reb = 1.0, rec = 0.0, red = 0.0;
@@ -1412,7 +1425,7 @@ __device__ __forceinline__ void start_imf(const SignalIO& io, SiftShared<WG>& sh
   const int N = io.N, nblk = ceil_div(N, 64);
   // block sums in the fourth row of the work area's sums: the end-condition reduction
   // that precedes this pass may still be reading the first
-  double* bs = wk.bsum + 3 * wk.nbcap;
+  EMD_ARR(double) bs = wk.bsum + 3 * wk.nbcap;
   double omax = -INFINITY, omin = INFINITY;
   for (int blk0 = w; blk0 < nblk; blk0 += kPrefetch * NW) {
     double sv[kPrefetch], av[kPrefetch];
@@ -1568,7 +1581,7 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
     // sum of squares and range are the next first sift's imf_old statistics.
     const bool fuse = extNo_last > 2;
     double* row = io.out + (int64_t)imf_count * N;
-    double* bs3 = wk.bsum + 3 * wk.nbcap;
+    EMD_ARR(double) bs3 = wk.bsum + 3 * wk.nbcap;
     double tmax = -INFINITY, tmin = INFINITY;
     for (int blk0 = w; blk0 < nblk; blk0 += kPrefetch * NW) {
       // the global loads of kPrefetch blocks are issued before the first use
@@ -1731,7 +1744,8 @@ __device__ __forceinline__ WgMem workgroup_mem(double* lds, char* spill, int64_t
   if (P.spill_slots > 0) {
     // solver rows after the knot tile (tile_bytes rounding: a multiple of 256 bytes)
     const int64_t tb = ((int64_t)P.spill_slots * kSlotBytes + 255) / 256 * 256;
-    m.tg.R = reinterpret_cast<double*>(region + tb);
+    double* rows = reinterpret_cast<double*>(region + tb);
+    m.tg.R = EMD_SPAN(double, rows, 8 * (int64_t)P.spill_rows, "tile.R");
     m.tg.rows = P.spill_rows;
   }
   m.tl.R = m.tg.R;
@@ -1741,9 +1755,8 @@ __device__ __forceinline__ WgMem workgroup_mem(double* lds, char* spill, int64_t
 
 // One workgroup per trace (grid = traces of this launch).  The reference launch model.
 template <int WG>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void emd_plain_kernel(Batch b, SiftParams P) {
-  extern __shared__ double dyn_lds[];
-  __shared__ SiftShared<WG> sh;
+__device__ __forceinline__ void plain_body(const Batch& b, const SiftParams& P,
+                                           SiftShared<WG>& sh, double* dyn_lds) {
   const WgMem m = workgroup_mem(dyn_lds, b.spill, b.spill_stride, P);
   const int q = b.base + (int)blockIdx.x;
   const int s = b.order ? b.order[q] : q;
@@ -1754,9 +1767,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void em
 // Persistent workgroups pull queue positions from an atomic counter until the b.B queue
 // entries are drained.
 template <int WG>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void emd_persistent_kernel(Batch b, SiftParams P) {
-  extern __shared__ double dyn_lds[];
-  __shared__ SiftShared<WG> sh;
+__device__ __forceinline__ void persistent_body(const Batch& b, const SiftParams& P,
+                                                SiftShared<WG>& sh, double* dyn_lds) {
   const WgMem m = workgroup_mem(dyn_lds, b.spill, b.spill_stride, P);
   while (true) {
     if (threadIdx.x == 0) sh.queue_idx = (int)atomicAdd(b.queue, 1u);
@@ -1769,6 +1781,20 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void em
     if ((unsigned)s >= (unsigned)b.total) continue;  // a queue entry naming no trace
     decompose<WG>(signal_io(b, s, P.max_imfs), sh, m.wk, m.tl, m.tg, P);
   }
+}
+
+template <int WG>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void emd_plain_kernel(Batch b, SiftParams P) {
+  extern __shared__ double dyn_lds[];
+  __shared__ SiftShared<WG> sh;
+  plain_body<WG>(b, P, sh, dyn_lds);
+}
+
+template <int WG>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void emd_persistent_kernel(Batch b, SiftParams P) {
+  extern __shared__ double dyn_lds[];
+  __shared__ SiftShared<WG> sh;
+  persistent_body<WG>(b, P, sh, dyn_lds);
 }
 
 // Envelope pass without sifting (diagnostic; reference envelope_batch_kernel).
