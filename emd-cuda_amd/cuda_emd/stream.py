@@ -84,8 +84,9 @@ class StreamingEMD:
                        for k in ("n_imfs", "overflow", "debug_n", "debug_extno", "debug_nzm",
                                  "stats", "row_offsets")}
                 done[slot].record(self.compute_stream)
-            # the pinned staging buffer is reused only after this chunk's copy finished
-            self.copy_stream.synchronize()
+            # no wait here: this slot's staging buffer is refilled only after drain(slot),
+            # which waits for the chunk's decomposition, and that waited for its copy; the
+            # next chunk's host staging overlaps this chunk's copy
             pending[slot] = (i, start, count, dev)
         for slot in range(nslot):
             drain(slot)
