@@ -92,7 +92,7 @@ struct SiftShared {
   // mirror results
   int k0[2], nk[2], kind[2], mtot;
   int xf[2], xl[2];  // first and last true knot position per side
-  int bad;
+  int bad, bad_m;    // wrong-signed knots: all of them / the mirrored ones
   double p3[2][4];
   double carry[2][3];  // sum of squares, max, min of the IMF left by sift n (slot n & 1)
   int queue_idx;
@@ -355,14 +355,14 @@ __device__ __forceinline__ void serial_extrema(const double* S, int N, bool want
   // a run still open here reaches the end of the signal and is dropped
 }
 
-template <int WG>
+// Blocks first, first + step, ... of the calling wave.
 __device__ __forceinline__ void compact_knots(const Work& wk, int N, const Tile& t, int nm,
-                                              int nn, int pad, uint64_t& bad) {
-  constexpr int NW = WG / 64;
-  const int tid = fresh_tid(), lane = tid & 63, w = uni(tid >> 6);
+                                              int nn, int pad, uint64_t& bad, int first,
+                                              int step) {
+  const int lane = fresh_tid() & 63;
   const int nblk = ceil_div(N, 64);
   const int tb0 = pad, tb1 = 3 * pad + nm;
-  for (int blk = w; blk < nblk; blk += NW) {
+  for (int blk = first; blk < nblk; blk += step) {
     const int i = blk * 64 + lane;
     // every load of the block is issued before the first use (one LDS round trip); the
     // masks stay in vector registers, so no scalar read waits for them
@@ -490,19 +490,7 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
   R.nn = min(R.tn, P.max_extrema);
   R.ovf = (R.tm >= P.max_extrema || R.tn >= P.max_extrema);
   R.in_lds = (4 * pad + R.nm + R.nn <= P.tile_slots);
-
-  // ---- pass B: compact the extrema into the knot tile ---------------------------------
-  uint64_t bad = 0;
-  if (R.in_lds) compact_knots<WG>(wk, N, tl, R.nm, R.nn, pad, bad);
-  else compact_knots<WG>(wk, N, tg, R.nm, R.nn, pad, bad);
-  const int bw = bad != 0ull ? 1 : 0;
-  if (lane == 0) sh.bad_w[w] = bw;
-  __syncthreads();
-  if (tid == 0) {
-    int b = 0;
-    for (int q = 0; q < NW; ++q) b |= sh.bad_w[q];
-    sh.bad = b;
-  }
+  // the knots are compacted into the tile by the sift that uses them (build_envelopes)
   return R;
 }
 
@@ -542,8 +530,8 @@ __device__ __forceinline__ int kd_slot(const KDesc& d, int q) {
   return d.rev ? d.to - 1 - q : d.from + q;
 }
 // The first and last nbsym+1 true knots of each side, which are all the mirror reads:
-// lanes 0..4F-1 of the mirroring wave each load one (position, value) in a single LDS
-// round trip, and the wave-uniform mirror logic fetches them with readlane.
+// lanes 0..4F-1 of the mirroring wave each hold one (position, value), and the
+// wave-uniform mirror logic fetches them with readlane.
 struct KnotEnds {
   int xi;      // this lane's cached knot position
   double yv;   // this lane's cached knot value
@@ -553,8 +541,32 @@ struct KnotEnds {
   double S0, SN;  // edge sample values
 };
 
-__device__ __forceinline__ KnotEnds load_ends(const Tile& t, int tb0, int tb1, int nm, int nn,
-                                              int F, int lane, int N, double S0, double SN) {
+// Position of set bit j (counting from 0 at the lowest) of m; j < popcount(m).
+__device__ __forceinline__ int nth_bit64(uint64_t m, int j) {
+  int pos = 0;
+#pragma unroll
+  for (int half = 32; half >= 1; half >>= 1) {
+    const uint64_t low = m & ((uint64_t(1) << half) - 1);
+    const int c = __popcll(low);
+    if (j >= c) {
+      j -= c;
+      m >>= half;
+      pos += half;
+    } else {
+      m = low;
+    }
+  }
+  return pos;
+}
+
+// The end knots straight from the extrema masks and the block prefix counts of the last
+// detection (the compaction that writes the tile runs at the same time): lane
+// L = side * 2F + half * F + k caches knot slot clamp(half ? n - F + k : k, 0, n - 1) of
+// its side.  Knot g of a side lies in the last block whose exclusive prefix is <= g (a
+// ballot over the blocks, whose prefixes do not decrease), at the set bit g - prefix of
+// the block's mask.  Three LDS round trips: prefixes, the lane's block, its sample.
+__device__ __forceinline__ KnotEnds ends_from_masks(const Work& wk, int nblk, int nm, int nn,
+                                                    int F, int N, double S0, double SN) {
   KnotEnds c;
   c.last = N - 1;
   c.S0 = S0;
@@ -564,17 +576,29 @@ __device__ __forceinline__ KnotEnds load_ends(const Tile& t, int tb0, int tb1, i
   c.n[1] = nn;
   c.xi = 0;
   c.yv = 0.0;
-  if (lane < 4 * F) {
-    // lane = side * 2F + half * F + k, by compares (F is a runtime value)
-    const int side = lane >= 2 * F ? 1 : 0;
-    const int r = lane - side * 2 * F;
-    const int half = r >= F ? 1 : 0;
-    const int k = r - half * F;
-    const int n = side ? nn : nm;
-    const int slot = max(0, min(half ? n - F + k : k, n - 1));
-    const int tb = side ? tb1 : tb0;
-    c.xi = t.X[tb + slot];
-    c.yv = t.Y[tb + slot];
+  const int lane = fresh_tid() & 63;
+  int myblk = 0, myg = 0, myside = 0;
+  for (int b0 = 0; b0 < nblk; b0 += 64) {
+    const int b = b0 + lane;
+    const int pm = b < nblk ? wk.pmax[b] : INT_MAX, pn = b < nblk ? wk.pmin[b] : INT_MAX;
+    for (int L = 0; L < 4 * F; ++L) {
+      const int side = L >= 2 * F ? 1 : 0;
+      const int r = L - side * 2 * F, half = r >= F ? 1 : 0, k = r - half * F;
+      const int n = side ? nn : nm;
+      const int g = max(0, min(half ? n - F + k : k, n - 1));
+      const uint64_t le = __ballot((side ? pn : pm) <= g);
+      if (le != 0ull && lane == L) {
+        myblk = b0 + 63 - __builtin_clzll(le);
+        myg = g;
+        myside = side;
+      }
+    }
+  }
+  if (lane < 4 * F && (myside ? nn : nm) > 0) {
+    const uint64_t m = myside ? wk.mmin[myblk] : wk.mmax[myblk];
+    const int pre = myside ? wk.pmin[myblk] : wk.pmax[myblk];
+    c.xi = myblk * 64 + nth_bit64(m, myg - pre);
+    c.yv = wk.imf[c.xi];
   }
   return c;
 }
@@ -622,8 +646,7 @@ __device__ __forceinline__ void side_counts(const KnotEnds& c, int side, int n, 
 __device__ __forceinline__ void finish_lists(const Tile& t, const KnotEnds& c, int tb0, int tb1,
                                              int nm, int nn, const KDesc& lmax, const KDesc& lmin,
                                              const KDesc& rmax, const KDesc& rmin, int lsym,
-                                             int rsym, int* k0, int* nk, int* kind, int& bad,
-                                             double (*p3)[4]) {
+                                             int rsym, int* k0, int* nk, int* kind, int& bad) {
   constexpr int kR = kMaxNbsym + 1;
   const int lane = fresh_tid() & 63;
   int nl[2], rs[2];
@@ -660,37 +683,35 @@ __device__ __forceinline__ void finish_lists(const Tile& t, const KnotEnds& c, i
       t.Y[tb + n + e - rss] = val;
     }
   }
-  if (kind[0] == kThreePoint || kind[1] == kThreePoint) wave_sync();
-#pragma unroll
-  for (int sd = 0; sd < 2; ++sd) {
-    if (kind[sd] != kThreePoint) continue;
-    // PyEMD cubic_spline_3pts: natural cubic spline in slope form, solved in closed form
-    const int q0 = k0[sd];
-    const double x0 = t.X[q0], x1 = t.X[q0 + 1], x2 = t.X[q0 + 2];
-    const double y0 = t.Y[q0], y1 = t.Y[q0 + 1], y2 = t.Y[q0 + 2];
-    const double h1 = x1 - x0, h2 = x2 - x1;
-    const double dy1 = y1 - y0, dy2 = y2 - y1;
-    const double i1 = 1.0 / h1, i2 = 1.0 / h2;
-    const double v1 = 3.0 * dy1 * i1 * i1;
-    const double v3 = 3.0 * dy2 * i2 * i2;
-    const double k1 = (v1 + v3) / (3.0 * (i1 + i2));
-    const double kk0 = (v1 - i1 * k1) / (2.0 * i1);
-    const double kk2 = (v3 - i2 * k1) / (2.0 * i2);
-    p3[sd][0] = kk0 * h1 - dy1;   // a1
-    p3[sd][1] = -k1 * h1 + dy1;   // b1
-    p3[sd][2] = k1 * h2 - dy2;    // a2
-    p3[sd][3] = -kk2 * h2 + dy2;  // b2
-  }
+}
+
+// PyEMD cubic_spline_3pts: the natural cubic spline through the three knots from slot q0,
+// in slope form, solved in closed form (coefficients a1, b1, a2, b2 of its two intervals).
+__device__ __forceinline__ void three_point_coeffs(const Tile& t, int q0, double* p3) {
+  const double x0 = t.X[q0], x1 = t.X[q0 + 1], x2 = t.X[q0 + 2];
+  const double y0 = t.Y[q0], y1 = t.Y[q0 + 1], y2 = t.Y[q0 + 2];
+  const double h1 = x1 - x0, h2 = x2 - x1;
+  const double dy1 = y1 - y0, dy2 = y2 - y1;
+  const double i1 = 1.0 / h1, i2 = 1.0 / h2;
+  const double v1 = 3.0 * dy1 * i1 * i1;
+  const double v3 = 3.0 * dy2 * i2 * i2;
+  const double k1 = (v1 + v3) / (3.0 * (i1 + i2));
+  const double kk0 = (v1 - i1 * k1) / (2.0 * i1);
+  const double kk2 = (v3 - i2 * k1) / (2.0 * i2);
+  p3[0] = kk0 * h1 - dy1;   // a1
+  p3[1] = -k1 * h1 + dy1;   // b1
+  p3[2] = k1 * h2 - dy2;    // a2
+  p3[3] = -kk2 * h2 + dy2;  // b2
 }
 
 // Boundary mirroring, run wave-uniformly by one wave (its results and knot writes are the
 // same in every lane; lane 0 writes).  PyEMD _prepare_points_simple with T = arange(N);
 // the reference's mirror_one_side (kernels.py:360-558) follows the same branch table.
-__device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int nb, double S0,
-                                       double SN, int* k0, int* nk, int* kind, int& bad,
-                                       double (*p3)[4], int* xf, int* xl) {
+__device__ __forceinline__ void mirror(const Tile& t, const Work& wk, int N, int nm, int nn,
+                                       int nb, double S0, double SN, int* k0, int* nk,
+                                       int* kind, int& bad, int* xf, int* xl) {
   const int tb0 = nb, tb1 = 3 * nb + nm;
-  const KnotEnds c = load_ends(t, tb0, tb1, nm, nn, nb + 1, fresh_tid() & 63, N, S0, SN);
+  const KnotEnds c = ends_from_masks(wk, ceil_div(N, 64), nm, nn, nb + 1, N, S0, SN);
   const int Xm0 = ends_x(c, 0, 0), XmL = ends_x(c, 0, nm - 1);
   const int Xn0 = ends_x(c, 1, 0), XnL = ends_x(c, 1, nn - 1);
   xf[0] = Xm0;
@@ -766,8 +787,7 @@ __device__ __forceinline__ void mirror(const Tile& t, int N, int nm, int nn, int
     else rmin = kd(max(nn - nb, 0), nn, 1);
     rsym = N - 1;
   }
-  finish_lists(t, c, tb0, tb1, nm, nn, lmax, lmin, rmax, rmin, lsym, rsym, k0, nk, kind, bad,
-               p3);
+  finish_lists(t, c, tb0, tb1, nm, nn, lmax, lmin, rmax, rmin, lsym, rsym, k0, nk, kind, bad);
 }
 
 // =======================================================================================
@@ -1356,28 +1376,29 @@ __device__ __forceinline__ bool check_imf(const SiftStats& s, int bad) {
   return false;
 }
 
-// Mirror + spline solve for the knots of detection D (shared by the sift and the
-// envelope-only kernel).
+// Knots of detection D into the tile, mirror, interval setup and spline solve (shared by the
+// sift and the envelope-only kernel).  Wave 0 mirrors (its end knots come from the extrema
+// masks) while the other waves compact the true extrema into the tile; after a barrier every
+// wave sets up the intervals between true knots, and waves 0 and 1 solve.
 template <int WG>
 __device__ __forceinline__ void build_envelopes(const Tile& t, SiftShared<WG>& sh,
                                                 const DetectResult& D, int N,
                                                 const SiftParams& P, const Work& wk) {
   constexpr int NW = WG / 64;
-  if (threadIdx.x < 64) {
+  const int tid = fresh_tid(), lane = tid & 63, w = uni(tid >> 6);
+  if (w == 0) {
     __builtin_amdgcn_s_setprio(2);  // the mirror is the longest single-wave step
-    int bad = sh.bad;
+    int bad = 0;
     int k0[2], nk[2], kind[2], xf[2], xl[2];
-    double p3[2][4];
-    mirror(t, N, D.nm, D.nn, P.nbsym, wk.imf[0], wk.imf[N - 1], k0, nk, kind, bad, p3, xf, xl);
-    if (threadIdx.x == 0) {
-      sh.bad = bad;
+    mirror(t, wk, N, D.nm, D.nn, P.nbsym, wk.imf[0], wk.imf[N - 1], k0, nk, kind, bad, xf, xl);
+    if (tid == 0) {
+      sh.bad_m = bad;
       for (int q = 0; q < 2; ++q) {
         sh.k0[q] = k0[q];
         sh.nk[q] = nk[q];
         sh.kind[q] = kind[q];
         sh.xf[q] = xf[q];
         sh.xl[q] = xl[q];
-        for (int r = 0; r < 4; ++r) sh.p3[q][r] = p3[q][r];
       }
       const int m0 = kind[0] == kNotAKnot ? nk[0] - 2 : 0;
       const int m1 = kind[1] == kNotAKnot ? nk[1] - 2 : 0;
@@ -1385,13 +1406,28 @@ __device__ __forceinline__ void build_envelopes(const Tile& t, SiftShared<WG>& s
     }
     __builtin_amdgcn_s_setprio(0);
   }
+  // the true extrema into the knot tile: waves 1.. while wave 0 mirrors (the slots are
+  // disjoint); wave 0 itself when it is the only one
+  if (NW == 1 || w > 0) {
+    uint64_t bad = 0;
+    compact_knots(wk, N, t, D.nm, D.nn, P.nbsym, bad, NW == 1 ? 0 : w - 1, NW == 1 ? 1 : NW - 1);
+    if (lane == 0) sh.bad_w[w] = bad != 0ull ? 1 : 0;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int b = sh.bad_m;
+    for (int q = NW == 1 ? 0 : 1; q < NW; ++q) b |= sh.bad_w[q];
+    sh.bad = b;
+#pragma unroll
+    for (int side = 0; side < 2; ++side)
+      if (sh.kind[side] == kThreePoint) three_point_coeffs(t, sh.k0[side], sh.p3[side]);
+  }
   // interval setup between consecutive true knots of both sides (the mirror does not move
-  // them): waves 1.. while wave 0 mirrors; wave 0 itself when it is the only one
-  if (NW == 1 || threadIdx.x >= 64) {
+  // them)
+  {
     const int nt0 = max(D.nm - 1, 0), nt = nt0 + max(D.nn - 1, 0);
     const int tb0 = P.nbsym, tb1 = 3 * P.nbsym + D.nm;
-    for (int q = NW == 1 ? threadIdx.x : threadIdx.x - 64; q < nt; q += (NW == 1 ? 64 : WG - 64))
-      interval_setup(t, q < nt0 ? tb0 + q : tb1 + (q - nt0));
+    for (int q = tid; q < nt; q += WG) interval_setup(t, q < nt0 ? tb0 + q : tb1 + (q - nt0));
   }
   __syncthreads();
   spline_solve<WG>(t, sh, D, P.nbsym);
