@@ -1549,6 +1549,7 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
 
   int imf_count = 0;
   bool finished = false;
+  bool res_in_lds = false;  // the residue is already in the work area (no HBM re-read)
   int extNo_last = -1;
   int overflow = 0;
   int sifts = 0, peak = 0, knot_sum = 0, flags = 0;
@@ -1697,13 +1698,17 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
     } else {
       have_acc = true;  // the accept pass did both
     }
-    if (leaving) break;  // PyEMD drops the trend row after leaving its loop
+    if (leaving) {
+      // after a fused accept pass the LDS holds S - acc, the residue, rounding included
+      res_in_lds = fuse;
+      break;  // PyEMD drops the trend row after leaving its loop
+    }
   }
 
   // residue = S - sum(IMFs); appended unless np.allclose(residue, 0) (max|r| <= 1e-8)
   int app = 0;
   for (int i = tid; i < N; i += WG) {
-    const double r = io.S[i] - (have_acc ? acc[i] : 0.0);
+    const double r = res_in_lds ? wk.imf[i] : io.S[i] - (have_acc ? acc[i] : 0.0);
     wk.imf[i] = r;
     app |= !(fabs(r) <= kResidueAtol);
   }
