@@ -66,7 +66,7 @@ struct emd_handle {
 
 // ---- Static scratch of emd_run_ragged_noalloc --------------------------------------------
 // One array per GPU, held until the process ends: the queue counter in its first 256
-// bytes, then the workgroups' global knot tiles and solver rows.  Every call on a GPU
+// bytes, then the workgroups' global knot tiles.  Every call on a GPU
 // shares it (calls that overlap on one GPU race); each call clears the counter first.
 namespace {
 constexpr size_t kStaticScratchBytes = (size_t)EMD_STATIC_SCRATCH_BYTES;
@@ -125,7 +125,7 @@ int cap_slots(const emd_config& c, int n) {
 }
 
 struct Shape {
-  int wg, tile, spill_slots, spill_rows, max_n;
+  int wg, tile, spill_slots, max_n;
 };
 
 int check_wg(int wg) {
@@ -157,14 +157,8 @@ int make_shape(const emd_handle* h, int wg, int tile_req, int max_n, Shape* s) {
   tile = std::max(0, std::min(tile, lds_fit_slots(h, wg, max_n)));
   s->wg = wg;
   s->tile = tile;
-  // A global region backs sifts whose knots outgrow the LDS tile, and holds the solver
-  // rows of systems longer than 64 * (kRegRows + 2) rows (a side has at most
-  // min(max_extrema, (max_n + 1) / 2) + 2 * nbsym knots).
-  // A system has m = knots - 2 <= max_rows - 2 rows, and its chunk rows are indexed
-  // below m, so max_rows rows per side and array are enough.
-  const int max_rows = std::min(h->cfg.max_extrema, (max_n + 1) / 2) + 2 * h->cfg.nbsym;
-  s->spill_slots = (tile < cap || max_rows > 64 * (emdk::kRegRows + 2)) ? cap : 0;
-  s->spill_rows = s->spill_slots ? max_rows : 0;
+  // a global knot tile of the full capacity backs sifts whose knots outgrow the LDS tile
+  s->spill_slots = tile < cap ? cap : 0;
   s->max_n = std::max(max_n, 1);
   return EMD_OK;
 }
@@ -179,7 +173,6 @@ emdk::SiftParams params(const emd_handle* h, const Shape& s, int zero_fill) {
   P.max_imfs = h->cfg.max_imfs;
   P.tile_slots = s.tile;
   P.spill_slots = s.spill_slots;
-  P.spill_rows = s.spill_rows;
   P.max_n = s.max_n;
   P.zero_fill = zero_fill;
   return P;
@@ -327,9 +320,7 @@ static int make_plan(const emd_handle* h, int B, int max_n, const emd_launch& L,
                       &pl->s);
   if (rc) return rc;
   pl->P = params(h, pl->s, zero_fill);
-  pl->stride = pl->s.spill_slots
-                   ? emdk::global_region_bytes(pl->s.spill_slots, pl->s.spill_rows)
-                   : 0;
+  pl->stride = pl->s.spill_slots ? emdk::tile_bytes(pl->s.spill_slots) : 0;
   pl->persistent = L.persistent ? 1 : 0;
   if (pl->persistent) {
     const size_t lds = emdk::dyn_lds_bytes(pl->s.tile, pl->P.max_n);
@@ -523,7 +514,7 @@ int emd_run_ragged_noalloc(emd_handle* h, const double* values, const int64_t* o
   if (pl.stride > 0) {
     if ((size_t)pl.stride > room)
       return fail(EMD_ERR_VALUE,
-                  "this launch needs %lld bytes of global knot tile and solver rows per "
+                  "this launch needs %lld bytes of global knot tile per "
                   "workgroup; the static scratch holds %zu (lower max_extrema or max_n, or "
                   "use emd_run_ragged)",
                   (long long)pl.stride, room);
@@ -554,8 +545,7 @@ int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double
   if (rc) return rc;
   const emdk::SiftParams P = params(h, s, 1);
   const int mbuf = h->cfg.max_extrema + 2 * h->cfg.nbsym;
-  const int64_t stride =
-      s.spill_slots ? emdk::global_region_bytes(s.spill_slots, s.spill_rows) : 0;
+  const int64_t stride = s.spill_slots ? emdk::tile_bytes(s.spill_slots) : 0;
   if (stride) {
     rc = ensure_spill(h, (size_t)stride * std::min(B, kMaxPlainBlocks));
     if (rc) return rc;

@@ -15,10 +15,6 @@ int64_t tile_bytes(int slots) {
   return (((int64_t)slots * kSlotBytes) + 255) / 256 * 256;
 }
 
-int64_t global_region_bytes(int slots, int rows) {
-  return tile_bytes(slots) + ((int64_t)rows * kRowBytes + 255) / 256 * 256;
-}
-
 int static_lds_bytes(int wg) {
   switch (wg) {
     case 64: return (int)sizeof(SiftShared<64>);

@@ -8,8 +8,6 @@ namespace emdk {
 constexpr int kMaxNbsym = 4;      // mirrored points per side supported by the kernels
 constexpr int kMaxN = 16384;      // longest trace (256 blocks of 64 samples)
 constexpr int kSlotBytes = 28;    // bytes per knot slot: position int32, value, M, 1/h fp64
-constexpr int kRegRows = 4;       // interior solver rows a lane keeps in registers
-constexpr int kRowBytes = 64;  // global solver rows: 2 sides x 4 fp64 per row
 constexpr int kWorkBlockBytes = 568;  // LDS bytes per 64-sample block of the work area
 
 struct SiftParams {
@@ -18,11 +16,8 @@ struct SiftParams {
   int max_iteration;  // sift cap per IMF
   int max_imfs;       // output rows per trace (IMF cap = max_imfs - 1, then the residue)
   int tile_slots;     // knot slots of the LDS tile (both envelopes)
-  int spill_slots;    // knot slots of each workgroup's global region (0: none): a global
-                      // knot tile that holds any trace, plus solver rows for systems
-                      // longer than 64 * (kRegRows + 2) rows
-  int spill_rows;     // solver rows per side held in the global region (>= the longest
-                      // system of the launch)
+  int spill_slots;    // knot slots of each workgroup's global knot tile (0: none), which
+                      // holds any trace's knots when the LDS tile does not
   int max_n;          // longest trace of the launch (sizes the LDS work area)
   int zero_fill;      // write zeros into output rows at and beyond n_imfs
 };
@@ -72,9 +67,6 @@ struct EnvBatch {
 
 // Bytes of one workgroup's global knot tile of `slots` slots.
 int64_t tile_bytes(int slots);
-// bytes of one workgroup's global region: a knot tile of `slots` slots, then the solver
-// rows (2 sides x 4 fp64 arrays of `rows` entries)
-int64_t global_region_bytes(int slots, int rows);
 // Static LDS of the sift kernels for a workgroup size.
 int static_lds_bytes(int wg);
 // Dynamic LDS of a launch: the per-trace work area (IMF + block bookkeeping) for traces of

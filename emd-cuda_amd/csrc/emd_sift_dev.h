@@ -62,10 +62,6 @@ struct Tile {
                       // overwritten by the solve with the second derivative at the slot
   EMD_ARR(double) H;  // 1/h of the interval that starts at each knot slot
   EMD_ARR(int) X;     // knot positions (sample index; mirrored knots may be <0 or >N-1)
-  EMD_ARR(double) R;  // solver rows of systems longer than 64 * (kRegRows + 2) rows, in the
-                      // workgroup's global region: side s, array a (g, u, v, w) at
-                      // R + (4 * s + a) * rows; null when the launch never needs them
-  int rows;
 };
 
 __device__ __forceinline__ Tile make_tile(double* base, int slots) {
@@ -75,8 +71,6 @@ __device__ __forceinline__ Tile make_tile(double* base, int slots) {
   t.H = EMD_SPAN(double, base + 2 * slots, slots, "tile.H");
   int* x = reinterpret_cast<int*>(base + 3 * slots);
   t.X = EMD_SPAN(int, x, slots, "tile.X");
-  t.R = nullptr;
-  t.rows = 0;
   return t;
 }
 
@@ -793,14 +787,15 @@ __device__ __forceinline__ void mirror(const Tile& t, const Work& wk, int N, int
 // =======================================================================================
 // Not-a-knot spline solves, one wave per envelope.  Each side's system has m = knots - 2
 // rows (M_0 and M_{n-1} eliminated) and is solved by a partition method: lane l owns a
-// contiguous chunk [s, e] of rows; a Thomas sweep over the chunk's interior expresses
-// every interior unknown as x = u - v*x_s - w*x_e; the chunk end rows then form a
-// tridiagonal system in (x_s, x_e) over all chunks, which an odd-even step reduces to
-// one unknown per lane and parallel cyclic reduction across lanes solves with shuffles.
-// The systems are diagonally dominant, so no pivoting is needed.  A lane keeps its chunk
-// rows in registers (up to kRegRows rows; longer systems keep them in the workgroup's
-// global region).  The solution goes to M by knot slot; the end second derivatives are
-// then recovered and 1/h is stored per interval.
+// contiguous chunk [s, e] of rows; a backward sweep over the chunk's interior expresses
+// x_{s+1} and x_{e-1} in terms of the chunk ends x_s and x_e; the chunk end rows then form
+// a tridiagonal system in (x_s, x_e) over all chunks, which an odd-even step reduces to
+// one unknown per lane and parallel cyclic reduction across lanes solves with shuffles;
+// a forward substitution then fills the chunk's interior.  The sweep parks its factors in
+// the chunk's own knot slots of H and M (no registers or global memory per row, so any
+// chunk length takes the same path), and the substitution overwrites them with the
+// solution and 1/h.  The systems are diagonally dominant, so no pivoting is needed.  The
+// solution goes to M by knot slot; the end second derivatives are then recovered.
 // =======================================================================================
 // Per interval k (knots k, k+1) of a not-a-knot side: H[k] = 1/h and, until the solve
 // overwrites it, M[k] = the chord slope (y_{k+1} - y_k) / h as (y_{k+1} - y_k) * (1/h).
@@ -810,13 +805,13 @@ __device__ __forceinline__ void interval_setup(const Tile& t, int k) {
   t.M[k] = (t.Y[k + 1] - t.Y[k]) * ih;
 }
 
-// Row j of the system of a side whose knots start at slot k0: a x_{j-1} + b x_j + c x_{j+1} = d.
-// Reads the chord slopes the interval setup left in M.
-__device__ __forceinline__ void nak_row(const Tile& t, int k0, int m, int j, double& a, double& b,
-                                        double& c, double& d) {
+// Row j of the system of a side whose knots start at slot k0: a x_{j-1} + b x_j + c x_{j+1} = d,
+// from the chord slopes sl, sr of the intervals left and right of knot k0+j+1.
+__device__ __forceinline__ void nak_row_sl(const Tile& t, int k0, int m, int j, double sl,
+                                           double sr, double& a, double& b, double& c,
+                                           double& d) {
   const int i = k0 + j + 1;
   const double xa = t.X[i - 1], xb = t.X[i], xc = t.X[i + 1];
-  const double sl = t.M[i - 1], sr = t.M[i];
   const double hl = xb - xa, hr = xc - xb;
   const double rhs = 6.0 * (sr - sl);
   if (j == 0) {
@@ -837,22 +832,98 @@ __device__ __forceinline__ void nak_row(const Tile& t, int k0, int m, int j, dou
   }
 }
 
-// One interior row of the chunk sweep: updates the carried (g, u, v, w) of row j from
-// those of row j-1.
-__device__ __forceinline__ void sweep_row(const Tile& t, int k0, int m, int j, bool first,
-                                          bool last, double& g, double& u, double& v,
-                                          double& w) {
+// The same row with the chord slopes the interval setup left in M.
+__device__ __forceinline__ void nak_row(const Tile& t, int k0, int m, int j, double& a, double& b,
+                                        double& c, double& d) {
+  const int i = k0 + j + 1;
+  nak_row_sl(t, k0, m, j, t.M[i - 1], t.M[i], a, b, c, d);
+}
+
+// The end rows of chunk [s, e] after eliminating its interior rows.  The chunk sweeps
+// backward from its end, x_j = p_j - q_j x_{j-1} - r_j x_e, carrying a rolling window of
+// knots and slopes; q_j goes to H[k0+j] and p_j to M[k0+j+1] (slots of this chunk that no
+// later row reads; the slope left of the end row, which the sweep overwrites, is kept in a
+// register), and the unrolled recurrence for x_{e-1} is summed on the way.  r_{s+1}, where
+// the substitution starts, is parked in M[k0+s+1] once the start row has read that slope.
+struct EndRows {
+  double sa, sb, sc, sd;  // start row: sa x_{s-1} + sb x_s + sc x_e = sd
+  double ea, eb, ec, ed;  // end row:   ea x_s + eb x_e + ec x_{e+1} = ed
+};
+
+__device__ __forceinline__ EndRows chunk_ends(const Tile& t, int k0, int m, int s, int e) {
+  const int len = e - s - 1;
+  const double sle = t.M[k0 + e];
+  int xm = t.X[k0 + e], xr = t.X[k0 + e + 1];
+  double sr = sle;
+  double p = 0.0, pq = 0.0, pr = 0.0, Q = 1.0, su = 0.0, sw = 0.0;
+  for (int q = 0; q < len; ++q) {
+    const int j = e - 1 - q;  // knots k0+j .. k0+j+2
+    const int xl = t.X[k0 + j];
+    const double sl = t.M[k0 + j];
+    const double hl = double(xm - xl), hr = double(xr - xm);
+    const double b = 2.0 * (hl + hr), d = 6.0 * (sr - sl);
+    const double rb = recip(q == 0 ? b : b - hr * pq);
+    p = (q == 0 ? d : d - hr * p) * rb;
+    pr = (q == 0 ? hr : -(hr * pr)) * rb;
+    pq = hl * rb;
+    t.H[k0 + j] = pq;
+    t.M[k0 + j + 1] = p;
+    su += Q * p;
+    sw += Q * pr;
+    Q = -(Q * pq);
+    xr = xm;
+    xm = xl;
+    sr = sl;
+  }
+  // x_{s+1} = p - pq x_s - pr x_e and x_{e-1} = su + Q x_s - sw x_e (with no interior
+  // rows the start row's x_{s+1} is x_e itself)
   double a, b, c, d;
-  nak_row(t, k0, m, j, a, b, c, d);
-  const double rb = recip(first ? b : b - a * g);
-  const double gn = last ? 0.0 : c * rb;
-  const double un = (first ? d : d - a * u) * rb;
-  const double vn = (first ? a : -(a * v)) * rb;
-  const double wn = ((last ? c : 0.0) - (first ? 0.0 : a * w)) * rb;
-  g = gn;
-  u = un;
-  v = vn;
-  w = wn;
+  EndRows r;
+  nak_row(t, k0, m, s, a, b, c, d);
+  r.sa = a;
+  r.sb = b - c * pq;
+  r.sc = len > 0 ? -(c * pr) : c;
+  r.sd = d - c * p;
+  if (len > 0) t.M[k0 + s + 1] = pr;
+  nak_row_sl(t, k0, m, e, sle, t.M[k0 + e + 1], a, b, c, d);
+  r.ea = a * Q;
+  r.eb = b - a * sw;
+  r.ec = c;
+  r.ed = d - a * su;
+  return r;
+}
+
+// The interior of chunk [s, e] once x_s and x_e are known: forward substitution over the
+// parked (q_j, p_j), x_j = p_j - q_j x_{j-1} - r_j x_e, with r_j rebuilt from r_{s+1} by
+// r_{j+1} = -r_j (b_j - c_j q_{j+1}) / c_j; x_j replaces p_j in M[k0+j+1], 1/h of interval
+// k0+j goes back into H[k0+j], and x_s, x_e are stored last.
+__device__ __forceinline__ void chunk_interior(const Tile& t, int k0, int s, int e, double xs,
+                                               double xe) {
+  const int len = e - s - 1;
+  if (len > 0) {
+    double x = xs, r = t.M[k0 + s + 1], qj = t.H[k0 + s + 1];
+    int xl = t.X[k0 + s + 1], xm = t.X[k0 + s + 2];
+    double ihl = recip(double(xm - xl));
+    for (int q = 0; q < len; ++q) {
+      const int j = s + 1 + q;  // knots k0+j .. k0+j+2
+      const int xr = t.X[k0 + j + 2];
+      const double hl = double(xm - xl), hr = double(xr - xm);
+      const double ihr = recip(hr);
+      x = t.M[k0 + j + 1] - qj * x - r * xe;
+      t.M[k0 + j + 1] = x;
+      if (q + 1 < len) {
+        const double qn = t.H[k0 + j + 1];
+        r = -(r * ((2.0 * (hl + hr) - hr * qn) * ihr));
+        qj = qn;
+      }
+      t.H[k0 + j] = ihl;
+      ihl = ihr;
+      xl = xm;
+      xm = xr;
+    }
+  }
+  t.M[k0 + 1 + s] = xs;
+  t.M[k0 + 1 + e] = xe;
 }
 
 // Solve one side's system (knots from slot k0, m rows) with the calling wave; every lane
@@ -864,86 +935,22 @@ __device__ __forceinline__ void spline_system_wave(const Tile& t, int side, int 
   // leaves chunks of 2 (plus one of 3 when m is odd), except m = 3 (one chunk of 3)
   const int base = m >= 128 ? (m >> 6) : (m == 3 ? 3 : 2);
   const int rem = m >= 128 ? (m & 63) : (m == 3 ? 0 : (m & 1));
-  const bool in_regs = base + (rem > 0 ? 1 : 0) - 2 <= kRegRows;  // interior rows per lane
   const bool act = lane < K;
   const int s = act ? lane * base + min(lane, rem) : 0;
   const int e = act ? s + base + (lane < rem ? 1 : 0) - 1 : 0;
-  const int len = act ? e - s - 1 : 0;  // interior rows of this chunk
-  double rg[kRegRows], ru[kRegRows], rv[kRegRows], rw[kRegRows];
-  EMD_ARR(double) Rg = t.R ? t.R + (4 * side + 0) * t.rows : nullptr;
-  EMD_ARR(double) Ru = t.R ? t.R + (4 * side + 1) * t.rows : nullptr;
-  EMD_ARR(double) Rv = t.R ? t.R + (4 * side + 2) * t.rows : nullptr;
-  EMD_ARR(double) Rw = t.R ? t.R + (4 * side + 3) * t.rows : nullptr;
   // chunk end rows after eliminating the interior (identity rows on idle lanes)
   double rsa = 0.0, rsb = 1.0, rsc = 0.0, rsd = 0.0;
   double rea = 0.0, reb = 1.0, rec = 0.0, red = 0.0;
   if (act) {
-    double a, b, c, d;
-    double us = 0.0, vs = 0.0, ws = 0.0, ue = 0.0, ve = 0.0, we = 0.0;
-    if (len > 0) {
-      double g = 0.0, u = 0.0, v = 0.0, w = 0.0;
-      if (in_regs) {
-#pragma unroll
-        for (int q = 0; q < kRegRows; ++q) {
-          if (q < len) {
-            sweep_row(t, k0, m, s + 1 + q, q == 0, q == len - 1, g, u, v, w);
-            rg[q] = g;
-            ru[q] = u;
-            rv[q] = v;
-            rw[q] = w;
-          }
-        }
-        ue = u;
-        ve = v;
-        we = w;
-#pragma unroll
-        for (int q = kRegRows - 2; q >= 0; --q) {
-          if (q < len - 1) {
-            ru[q] = ru[q] - rg[q] * ru[q + 1];
-            rv[q] = rv[q] - rg[q] * rv[q + 1];
-            rw[q] = rw[q] - rg[q] * rw[q + 1];
-          }
-        }
-        us = ru[0];
-        vs = rv[0];
-        ws = rw[0];
-      } else {
-        for (int q = 0; q < len; ++q) {
-          sweep_row(t, k0, m, s + 1 + q, q == 0, q == len - 1, g, u, v, w);
-          Rg[s + 1 + q] = g;
-          Ru[s + 1 + q] = u;
-          Rv[s + 1 + q] = v;
-          Rw[s + 1 + q] = w;
-        }
-        ue = u;
-        ve = v;
-        we = w;
-        for (int q = len - 2; q >= 0; --q) {
-          const int j = s + 1 + q;
-          const double gq = Rg[j];
-          u = Ru[j] - gq * u;
-          v = Rv[j] - gq * v;
-          w = Rw[j] - gq * w;
-          Ru[j] = u;
-          Rv[j] = v;
-          Rw[j] = w;
-        }
-        us = u;
-        vs = v;
-        ws = w;
-      }
-    }
-    const bool interior = len > 0;
-    nak_row(t, k0, m, s, a, b, c, d);
-    rsa = a;
-    rsb = interior ? b - c * vs : b;
-    rsc = interior ? -(c * ws) : c;
-    rsd = interior ? d - c * us : d;
-    nak_row(t, k0, m, e, a, b, c, d);
-    rea = interior ? -(a * ve) : a;
-    reb = interior ? b - a * we : b;
-    rec = c;
-    red = interior ? d - a * ue : d;
+    const EndRows r = chunk_ends(t, k0, m, s, e);
+    rsa = r.sa;
+    rsb = r.sb;
+    rsc = r.sc;
+    rsd = r.sd;
+    rea = r.ea;
+    reb = r.eb;
+    rec = r.ec;
+    red = r.ed;
   }
   // odd-even step: eliminate x_s of this chunk and of the next from the e-row
   const double nsa = dpp_d<kDppWaveShl1>(rsa), nsb = dpp_d<kDppWaveShl1>(rsb);
@@ -994,18 +1001,7 @@ __device__ __forceinline__ void spline_system_wave(const Tile& t, int side, int 
   const double xprev = dpp_d<kDppWaveShr1>(xe);
   if (act) {
     const double xs = (rsd - (lane > 0 ? rsa * xprev : 0.0) - rsc * xe) * recip(rsb);
-    t.M[k0 + 1 + s] = xs;
-    t.M[k0 + 1 + e] = xe;
-    if (in_regs) {
-#pragma unroll
-      for (int q = 0; q < kRegRows; ++q)
-        if (q < len) t.M[k0 + 2 + s + q] = ru[q] - rv[q] * xs - rw[q] * xe;
-    } else {
-      for (int q = 0; q < len; ++q) {
-        const int j = s + 1 + q;
-        t.M[k0 + 1 + j] = Ru[j] - Rv[j] * xs - Rw[j] * xe;
-      }
-    }
+    chunk_interior(t, k0, s, e, xs, xe);
   }
 }
 
@@ -1782,15 +1778,6 @@ __device__ __forceinline__ WgMem workgroup_mem(double* lds, char* spill, int64_t
   m.tl = make_tile(lds + (int64_t)nbcap * (kWorkBytesPerBlock / 8), P.tile_slots);
   char* region = spill + (int64_t)blockIdx.x * stride;
   m.tg = make_tile(reinterpret_cast<double*>(region), P.spill_slots);
-  if (P.spill_slots > 0) {
-    // solver rows after the knot tile (tile_bytes rounding: a multiple of 256 bytes)
-    const int64_t tb = ((int64_t)P.spill_slots * kSlotBytes + 255) / 256 * 256;
-    double* rows = reinterpret_cast<double*>(region + tb);
-    m.tg.R = EMD_SPAN(double, rows, 8 * (int64_t)P.spill_rows, "tile.R");
-    m.tg.rows = P.spill_rows;
-  }
-  m.tl.R = m.tg.R;
-  m.tl.rows = m.tg.rows;
   return m;
 }
 

@@ -100,7 +100,7 @@ int emd_run_ragged(emd_handle* h, const double* values, const int64_t* offsets, 
  * loops, harnesses that forbid hipMalloc).  Same arguments, outputs and queue semantics
  * as emd_run_ragged; the caller passes every input and output.  Instead of the handle's
  * scratch (grown by hipMalloc on demand), the queue counter and the workgroups' global
- * knot tiles and solver rows live in one static device array of EMD_STATIC_SCRATCH_BYTES
+ * knot tiles live in one static device array of EMD_STATIC_SCRATCH_BYTES
  * per GPU, held until the process ends and looked up by emd_create.  A launch whose
  * workgroups would need more than the array holds runs with fewer resident workgroups
  * (or shorter plain sub-launches), which never changes results; one whose single

@@ -8,7 +8,7 @@
 // Test infrastructure: tests/test_sift_emu.py writes a batch, runs this program on it and
 // compares the outputs with the oracle.  It runs the launch the C ABI would plan for the
 // same arguments (tile clamped to the capacity rule and to what 160 KB of LDS holds, a
-// global region of the full capacity when the tile is smaller, the same solver-row count),
+// global knot tile of the full capacity when the LDS tile is smaller),
 // with the number of concurrently resident workgroups given on the command line.
 //
 // usage: sift_emu <input file> <output file>
@@ -25,9 +25,8 @@
 
 namespace {
 
-int64_t region_bytes(int slots, int rows) {
-  auto r256 = [](int64_t b) { return (b + 255) / 256 * 256; };
-  return r256((int64_t)slots * emdk::kSlotBytes) + r256((int64_t)rows * emdk::kRowBytes);
+int64_t region_bytes(int slots) {
+  return ((int64_t)slots * emdk::kSlotBytes + 255) / 256 * 256;
 }
 
 size_t dyn_lds(int tile_slots, int max_n) {
@@ -150,8 +149,8 @@ int main(int argc, char** argv) {
   int max_n = 0;
   for (int s = 0; s < in.B; ++s) max_n = std::max<int>(max_n, in.offsets[s + 1] - in.offsets[s]);
   // the C ABI's plan (emd_capi.hip make_shape): capacity rule, tile clamped to it and to
-  // the LDS next to the work area, a global region of the full capacity behind a smaller
-  // tile or for systems longer than the register rows
+  // the LDS next to the work area, a global knot tile of the full capacity behind a
+  // smaller tile
   const int cap = 2 * std::min(in.max_extrema, std::max(max_n, 1)) + 4 * in.nbsym;
   int tile = in.tile > 0 ? std::min(in.tile, cap) : cap;
   const long lds_max = 163840;
@@ -161,18 +160,16 @@ int main(int argc, char** argv) {
                     (long)dyn_lds(0, max_n)) /
                    emdk::kSlotBytes;
   tile = std::max(0, std::min<int>(tile, (int)std::max(0L, fit)));
-  const int max_rows = std::min(in.max_extrema, (max_n + 1) / 2) + 2 * in.nbsym;
   emdk::SiftParams P;
   P.max_extrema = in.max_extrema;
   P.nbsym = in.nbsym;
   P.max_iteration = in.max_iteration;
   P.max_imfs = in.max_imfs;
   P.tile_slots = tile;
-  P.spill_slots = (tile < cap || max_rows > 64 * (emdk::kRegRows + 2)) ? cap : 0;
-  P.spill_rows = P.spill_slots ? max_rows : 0;
+  P.spill_slots = tile < cap ? cap : 0;
   P.max_n = std::max(max_n, 1);
   P.zero_fill = 0;
-  const int64_t stride = P.spill_slots ? region_bytes(P.spill_slots, P.spill_rows) : 0;
+  const int64_t stride = P.spill_slots ? region_bytes(P.spill_slots) : 0;
 
   const int K = in.max_imfs;
   std::vector<double> imfs((size_t)std::max<int64_t>(n_values * K, 1),
@@ -195,9 +192,9 @@ int main(int argc, char** argv) {
   b.dbg_e = de.data();
   b.dbg_z = dz.data();
   b.stats = stats.data();
-  std::fprintf(stderr, "sift_emu: B=%d wg=%d tile=%d (cap %d) spill_slots=%d rows=%d "
+  std::fprintf(stderr, "sift_emu: B=%d wg=%d tile=%d (cap %d) spill_slots=%d "
                "resident=%d persistent=%d max_n=%d\n", in.B, in.wg, tile, cap, P.spill_slots,
-               P.spill_rows, in.resident, in.persistent, max_n);
+               in.resident, in.persistent, max_n);
   switch (in.wg) {
     case 64: run<64>(in, b, P, stride); break;
     case 128: run<128>(in, b, P, stride); break;

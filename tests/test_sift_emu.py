@@ -9,6 +9,7 @@ enough that every sift spills to the workgroup's global tile, a random queue ord
 signals, and hold the outputs to the same parity bar as the GPU tests.  The three calls that
 hung on the GPU in earlier builds (tests/test_gpu_identity.py, skipped there) are replayed
 call by call, with the same workgroup sizes, tiles, queue orders and resident grids."""
+import fcntl
 import os
 import subprocess
 import sys
@@ -28,8 +29,11 @@ MAGIC = 0x454D5531
 
 @pytest.fixture(scope="module")
 def emu():
-    if not os.path.exists(EMU):
-        subprocess.run(["make", "-C", NATIVE, "sift_emu"], check=True)
+    # make rebuilds when the kernel source is newer; the lock keeps parallel test workers
+    # from building at the same time
+    with open(os.path.join(NATIVE, ".sift_emu.lock"), "w") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        subprocess.run(["make", "-s", "-C", NATIVE, "sift_emu"], check=True)
     return EMU
 
 
@@ -96,8 +100,8 @@ def test_plain_and_persistent_match_oracle(emu):
 
 
 def test_global_tile_and_random_order(emu):
-    """A 40-slot LDS tile sends every sift's knots to the workgroup's global tile (and the
-    long systems' solver rows); ragged lengths and a random order share one queue."""
+    """A 40-slot LDS tile sends every sift's knots to the workgroup's global tile; ragged
+    lengths and a random order share one queue."""
     from cuda_emd import synth
     rng = np.random.default_rng(3)
     base = synth.chirp_mix(4, 520, seed=9)
@@ -158,3 +162,15 @@ def test_replay_persistent_ragged_seismic_chirp_small_tile(emu):
     order = rng.permutation(24).astype(np.int32)
     out = run_emu(emu, traces, 64, tile=200, resident=6, persistent=True, order=order)
     _assert_parity(traces, out, "emu:replay-seismic-chirp")
+
+
+def test_long_spline_systems(emu):
+    """White-noise traces of 1600 samples: each side's spline system has hundreds of rows,
+    so every lane's chunk has many interior rows whose factors the sweep parks in the
+    chunk's knot slots."""
+    rng = np.random.default_rng(7)
+    traces = [rng.standard_normal(1600) for _ in range(2)]
+    out = run_emu(emu, traces, 128, resident=2, persistent=True)
+    _assert_parity(traces, out, "emu:long-systems")
+    ref = run_emu(emu, traces, 64, resident=2, persistent=False)
+    _same(out, ref)
