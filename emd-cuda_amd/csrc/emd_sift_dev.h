@@ -856,10 +856,16 @@ __device__ __forceinline__ EndRows chunk_ends(const Tile& t, int k0, int m, int 
   int xm = t.X[k0 + e], xr = t.X[k0 + e + 1];
   double sr = sle;
   double p = 0.0, pq = 0.0, pr = 0.0, Q = 1.0, su = 0.0, sw = 0.0;
+  // each row's left knot and slope are loaded one row ahead (the last such load is of the
+  // start row's, always a slot of this chunk)
+  int xl_n = t.X[k0 + e - 1];
+  double sl_n = t.M[k0 + e - 1];
   for (int q = 0; q < len; ++q) {
     const int j = e - 1 - q;  // knots k0+j .. k0+j+2
-    const int xl = t.X[k0 + j];
-    const double sl = t.M[k0 + j];
+    const int xl = xl_n;
+    const double sl = sl_n;
+    xl_n = t.X[k0 + j - 1];
+    sl_n = t.M[k0 + j - 1];
     const double hl = double(xm - xl), hr = double(xr - xm);
     const double b = 2.0 * (hl + hr), d = 6.0 * (sr - sl);
     const double rb = recip(q == 0 ? b : b - hr * pq);
@@ -904,15 +910,22 @@ __device__ __forceinline__ void chunk_interior(const Tile& t, int k0, int s, int
     double x = xs, r = t.M[k0 + s + 1], qj = t.H[k0 + s + 1];
     int xl = t.X[k0 + s + 1], xm = t.X[k0 + s + 2];
     double ihl = recip(double(xm - xl));
+    // each row's right knot, parked p and next q are loaded one row ahead (the last such
+    // loads reach knot k0+e+2 and slot k0+e+1, still this side's)
+    int xr_n = t.X[k0 + s + 3];
+    double p_n = t.M[k0 + s + 2], q_n = t.H[k0 + s + 2];
     for (int q = 0; q < len; ++q) {
       const int j = s + 1 + q;  // knots k0+j .. k0+j+2
-      const int xr = t.X[k0 + j + 2];
+      const int xr = xr_n;
+      const double pj = p_n, qn = q_n;
+      xr_n = t.X[k0 + j + 3];
+      p_n = t.M[k0 + j + 2];
+      q_n = t.H[k0 + j + 2];
       const double hl = double(xm - xl), hr = double(xr - xm);
       const double ihr = recip(hr);
-      x = t.M[k0 + j + 1] - qj * x - r * xe;
+      x = pj - qj * x - r * xe;
       t.M[k0 + j + 1] = x;
       if (q + 1 < len) {
-        const double qn = t.H[k0 + j + 1];
         r = -(r * ((2.0 * (hl + hr) - hr * qn) * ihr));
         qj = qn;
       }
