@@ -126,6 +126,7 @@ int cap_slots(const emd_config& c, int n) {
 
 struct Shape {
   int wg, tile, spill_slots, max_n;
+  int slot;  // bytes per knot slot: kSlotBytes, or kSlotBytesTime for the time-axis kernels
 };
 
 int check_wg(int wg) {
@@ -135,16 +136,18 @@ int check_wg(int wg) {
   return EMD_OK;
 }
 
-// LDS knot slots that fit next to the work area of traces up to max_n samples
-int lds_fit_slots(const emd_handle* h, int wg, int max_n) {
+// LDS knot slots of slot_bytes that fit next to the work area of traces up to max_n samples
+int lds_fit_slots(const emd_handle* h, int wg, int max_n, int slot_bytes = emdk::kSlotBytes) {
   const long avail =
       (long)h->lds_max - emdk::static_lds_bytes(wg) - (long)emdk::dyn_lds_bytes(0, max_n);
-  return avail > 0 ? (int)(avail / emdk::kSlotBytes) : 0;
+  return avail > 0 ? (int)(avail / slot_bytes) : 0;
 }
 
 // tile_req LDS knot slots (0 = the capacity rule), clamped to what LDS holds; a global
-// tile of the full capacity backs sifts whose knots do not fit.
-int make_shape(const emd_handle* h, int wg, int tile_req, int max_n, Shape* s) {
+// tile of the full capacity backs sifts whose knots do not fit.  timed: the time-axis
+// kernels (wider knot slots).
+int make_shape(const emd_handle* h, int wg, int tile_req, int max_n, Shape* s,
+               int timed = 0) {
   int rc = check_wg(wg);
   if (rc) return rc;
   if (max_n > emdk::kMaxN)
@@ -153,9 +156,11 @@ int make_shape(const emd_handle* h, int wg, int tile_req, int max_n, Shape* s) {
   if (emdk::dyn_lds_bytes(0, max_n) + emdk::static_lds_bytes(wg) > (size_t)h->lds_max)
     return fail(EMD_ERR_VALUE, "trace length %d does not fit the LDS work area", max_n);
   const int cap = cap_slots(h->cfg, max_n);
+  const int slot = timed ? emdk::kSlotBytesTime : emdk::kSlotBytes;
   int tile = tile_req > 0 ? std::min(tile_req, cap) : cap;
-  tile = std::max(0, std::min(tile, lds_fit_slots(h, wg, max_n)));
+  tile = std::max(0, std::min(tile, lds_fit_slots(h, wg, max_n, slot)));
   s->wg = wg;
+  s->slot = slot;
   s->tile = tile;
   // a global knot tile of the full capacity backs sifts whose knots outgrow the LDS tile
   s->spill_slots = tile < cap ? cap : 0;
@@ -315,16 +320,16 @@ struct Plan {
 };
 
 static int make_plan(const emd_handle* h, int B, int max_n, const emd_launch& L, int zero_fill,
-                     Plan* pl) {
+                     Plan* pl, int timed = 0) {
   int rc = make_shape(h, L.threads, L.tile_slots > 0 ? L.tile_slots : h->cfg.tile_slots, max_n,
-                      &pl->s);
+                      &pl->s, timed);
   if (rc) return rc;
   pl->P = params(h, pl->s, zero_fill);
-  pl->stride = pl->s.spill_slots ? emdk::tile_bytes(pl->s.spill_slots) : 0;
+  pl->stride = pl->s.spill_slots ? emdk::tile_bytes(pl->s.spill_slots, pl->s.slot) : 0;
   pl->persistent = L.persistent ? 1 : 0;
   if (pl->persistent) {
-    const size_t lds = emdk::dyn_lds_bytes(pl->s.tile, pl->P.max_n);
-    const int fit = emdk::max_resident_wgs(1, pl->s.wg, lds);
+    const size_t lds = emdk::dyn_lds_bytes(pl->s.tile, pl->P.max_n, pl->s.slot);
+    const int fit = emdk::max_resident_wgs(1, pl->s.wg, lds, timed);
     if (fit <= 0) return fail(EMD_ERR_RUNTIME, "persistent kernel shape does not fit a CU");
     int per_cu = L.wgs_per_cu;
     if (per_cu <= 0 || per_cu > fit) per_cu = fit;
@@ -408,7 +413,7 @@ static emdk::Batch make_batch(const double* values, const int64_t* offsets, int 
                               const int* order, int total, double* imfs,
                               const int64_t* out_offsets, int* n_imfs, int* overflow,
                               int* debug_n, int* debug_extno, int* debug_nzm, int* stats) {
-  emdk::Batch b;
+  emdk::Batch b{};
   b.values = values;
   b.offsets = offsets;
   b.N = N;
@@ -425,21 +430,26 @@ static emdk::Batch make_batch(const double* values, const int64_t* offsets, int 
   return b;
 }
 
+// times: null (positions = sample indices), or the time-axis kernels' normalised times,
+// laid out like values, or one shared row when times_shared.
 static int run_common(emd_handle* h, const double* values, const int64_t* offsets, int N,
                       int B, int max_n, const int* order, const emd_launch& L, double* imfs,
                       const int64_t* out_offsets, int* n_imfs, int* overflow, int* debug_n,
-                      int* debug_extno, int* debug_nzm, int* stats, hipStream_t st) {
+                      int* debug_extno, int* debug_nzm, int* stats, hipStream_t st,
+                      const double* times = nullptr, int times_shared = 0) {
   int rc = check_io(B, max_n, values, imfs, n_imfs, overflow, debug_n, debug_extno, debug_nzm);
   if (rc) return rc;
   h->timed = 0;
   if (B == 0) return EMD_OK;
   if (L.traces < 0) return fail(EMD_ERR_VALUE, "launch.traces must be >= 0, got %d", L.traces);
   Plan pl;
-  if ((rc = make_plan(h, B, max_n, L, offsets == nullptr, &pl))) return rc;
+  if ((rc = make_plan(h, B, max_n, L, offsets == nullptr, &pl, times != nullptr))) return rc;
   if (pl.spill_bytes && (rc = ensure_spill(h, pl.spill_bytes))) return rc;
-  const emdk::Batch b = make_batch(values, offsets, N, order, L.traces > 0 ? L.traces : B,
-                                   imfs, out_offsets, n_imfs, overflow, debug_n, debug_extno,
-                                   debug_nzm, stats);
+  emdk::Batch b = make_batch(values, offsets, N, order, L.traces > 0 ? L.traces : B, imfs,
+                             out_offsets, n_imfs, overflow, debug_n, debug_extno, debug_nzm,
+                             stats);
+  b.times = times;
+  b.times_shared = times_shared ? 1 : 0;
   if ((rc = scratch_acquire(h, st))) return rc;
   rc = enqueue(h, pl, b, B, h->spill, h->counters, h->timing != 0, st);
   const int rc2 = scratch_release(h, st);
@@ -482,6 +492,50 @@ int emd_run_ragged(emd_handle* h, const double* values, const int64_t* offsets, 
   }
   return run_common(h, values, offsets, 0, B, max_n, order, L, imfs, out_offsets, n_imfs,
                     overflow, debug_n, debug_extno, debug_nzm, stats, (hipStream_t)stream);
+}
+
+int emd_run_padded_times(emd_handle* h, const double* signals, const double* times,
+                         int times_per_signal, int B, int N, double* imfs, int* n_imfs,
+                         int* overflow, int* debug_n, int* debug_extno, int* debug_nzm,
+                         int* stats, void* stream) {
+  if (!h) return fail(EMD_ERR_VALUE, "null handle");
+  if (N < 0 || N > emdk::kMaxN)
+    return fail(EMD_ERR_VALUE, "trace length N=%d outside [0, %d]", N, emdk::kMaxN);
+  if (B > 0 && N > 0 && !times) return fail(EMD_ERR_VALUE, "null time axis");
+  emd_launch L;
+  L.threads = h->cfg.threads_per_block;
+  L.tile_slots = h->cfg.tile_slots;
+  L.wgs_per_cu = 0;
+  L.persistent = 0;
+  L.traces = 0;
+  return run_common(h, signals, nullptr, N, B, N, nullptr, L, imfs, nullptr, n_imfs, overflow,
+                    debug_n, debug_extno, debug_nzm, stats, (hipStream_t)stream, times,
+                    !times_per_signal);
+}
+
+int emd_run_ragged_times(emd_handle* h, const double* values, const double* times,
+                         const int64_t* offsets, int B, int max_n, const int* order,
+                         const emd_launch* launch, double* imfs, const int64_t* out_offsets,
+                         int* n_imfs, int* overflow, int* debug_n, int* debug_extno,
+                         int* debug_nzm, int* stats, void* stream) {
+  if (!h) return fail(EMD_ERR_VALUE, "null handle");
+  if (!offsets) return fail(EMD_ERR_VALUE, "ragged input needs CSR offsets");
+  if (max_n < 0 || max_n > emdk::kMaxN)
+    return fail(EMD_ERR_VALUE, "max trace length %d outside [0, %d]", max_n, emdk::kMaxN);
+  if (B > 0 && max_n > 0 && !times) return fail(EMD_ERR_VALUE, "null time axis");
+  emd_launch L;
+  if (launch) {
+    L = *launch;
+  } else {
+    L.threads = h->cfg.threads_per_block;
+    L.tile_slots = h->cfg.tile_slots;
+    L.wgs_per_cu = 0;
+    L.persistent = 1;
+    L.traces = 0;
+  }
+  return run_common(h, values, offsets, 0, B, max_n, order, L, imfs, out_offsets, n_imfs,
+                    overflow, debug_n, debug_extno, debug_nzm, stats, (hipStream_t)stream, times,
+                    0);
 }
 
 int emd_run_ragged_noalloc(emd_handle* h, const double* values, const int64_t* offsets, int B,
@@ -531,21 +585,25 @@ int emd_run_ragged_noalloc(emd_handle* h, const double* values, const int64_t* o
                  (hipStream_t)stream);
 }
 
-int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double* max_env,
-                      double* min_env, int* n_extrema, int* degenerate, double* max_kpos,
-                      double* max_kval, double* min_kpos, double* min_kval, int* n_mirrored,
-                      void* stream) {
+}  // extern "C"
+
+// The envelope-only pass, with positions from times (null: sample indices; else [B,N]
+// rows, or one shared [N] row when times_shared).
+static int envelope_common(emd_handle* h, const double* signals, const double* times,
+                           int times_shared, int B, int N, double* max_env, double* min_env,
+                           int* n_extrema, int* degenerate, double* max_kpos, double* max_kval,
+                           double* min_kpos, double* min_kval, int* n_mirrored, void* stream) {
   if (!h) return fail(EMD_ERR_VALUE, "null handle");
   if (N < 0 || N > emdk::kMaxN)
     return fail(EMD_ERR_VALUE, "trace length N=%d outside [0, %d]", N, emdk::kMaxN);
   if (B < 0) return fail(EMD_ERR_VALUE, "batch size must be >= 0, got %d", B);
   if (B == 0) return EMD_OK;
   Shape s;
-  int rc = make_shape(h, 256, h->cfg.tile_slots, N, &s);
+  int rc = make_shape(h, 256, h->cfg.tile_slots, N, &s, times != nullptr);
   if (rc) return rc;
   const emdk::SiftParams P = params(h, s, 1);
   const int mbuf = h->cfg.max_extrema + 2 * h->cfg.nbsym;
-  const int64_t stride = s.spill_slots ? emdk::tile_bytes(s.spill_slots) : 0;
+  const int64_t stride = s.spill_slots ? emdk::tile_bytes(s.spill_slots, s.slot) : 0;
   if (stride) {
     rc = ensure_spill(h, (size_t)stride * std::min(B, kMaxPlainBlocks));
     if (rc) return rc;
@@ -553,8 +611,10 @@ int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double
   if ((rc = scratch_acquire(h, (hipStream_t)stream))) return rc;
   for (int base = 0; base < B; base += kMaxPlainBlocks) {
     const int nb = std::min(kMaxPlainBlocks, B - base);
-    emdk::EnvBatch e;
+    emdk::EnvBatch e{};
     e.values = signals + (int64_t)base * N;
+    e.times = times ? times + (times_shared ? 0 : (int64_t)base * N) : nullptr;
+    e.times_shared = times_shared ? 1 : 0;
     e.N = N;
     e.B = nb;
     e.max_env = max_env + (int64_t)base * N;
@@ -575,6 +635,27 @@ int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double
     if (rc) return rc;
   }
   return scratch_release(h, (hipStream_t)stream);
+}
+
+extern "C" {
+
+int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double* max_env,
+                      double* min_env, int* n_extrema, int* degenerate, double* max_kpos,
+                      double* max_kval, double* min_kpos, double* min_kval, int* n_mirrored,
+                      void* stream) {
+  return envelope_common(h, signals, nullptr, 0, B, N, max_env, min_env, n_extrema, degenerate,
+                         max_kpos, max_kval, min_kpos, min_kval, n_mirrored, stream);
+}
+
+int emd_envelope_only_times(emd_handle* h, const double* signals, const double* times,
+                            int times_per_signal, int B, int N, double* max_env,
+                            double* min_env, int* n_extrema, int* degenerate, double* max_kpos,
+                            double* max_kval, double* min_kpos, double* min_kval,
+                            int* n_mirrored, void* stream) {
+  if (B > 0 && N > 0 && !times) return fail(EMD_ERR_VALUE, "null time axis");
+  return envelope_common(h, signals, times, !times_per_signal, B, N, max_env, min_env,
+                         n_extrema, degenerate, max_kpos, max_kval, min_kpos, min_kval,
+                         n_mirrored, stream);
 }
 
 }  // extern "C"

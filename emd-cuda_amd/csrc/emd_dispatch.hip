@@ -11,8 +11,8 @@ void* sift_instance_256(int which);
 void* sift_instance_512(int which);
 void* sift_instance_1024(int which);
 
-int64_t tile_bytes(int slots) {
-  return (((int64_t)slots * kSlotBytes) + 255) / 256 * 256;
+int64_t tile_bytes(int slots, int slot_bytes) {
+  return (((int64_t)slots * slot_bytes) + 255) / 256 * 256;
 }
 
 int static_lds_bytes(int wg) {
@@ -26,9 +26,9 @@ int static_lds_bytes(int wg) {
   return 0;
 }
 
-size_t dyn_lds_bytes(int tile_slots, int max_n) {
+size_t dyn_lds_bytes(int tile_slots, int max_n, int slot_bytes) {
   const size_t nbcap = (size_t)(max_n + 63) / 64;
-  return nbcap * kWorkBlockBytes + (size_t)tile_slots * kSlotBytes;
+  return nbcap * kWorkBlockBytes + (size_t)tile_slots * slot_bytes;
 }
 
 static void* instance(int which, int wg) {
@@ -42,15 +42,18 @@ static void* instance(int which, int wg) {
   return nullptr;
 }
 
-static void* sift_kernel_ptr(int persistent, int wg) { return instance(persistent ? 1 : 0, wg); }
-static void* envelope_kernel_ptr(int wg) { return instance(2, wg); }
+// timed: the instance whose positions come from a time axis
+static void* sift_kernel_ptr(int persistent, int wg, int timed) {
+  return instance((persistent ? 1 : 0) + (timed ? 3 : 0), wg);
+}
+static void* envelope_kernel_ptr(int wg, int timed) { return instance(timed ? 5 : 2, wg); }
 
 int prepare_kernels(int lds_per_workgroup) {
   // Allow each kernel the whole per-workgroup LDS once, so launches (possibly under
   // stream capture) never change function attributes.
   for (int wg : {64, 128, 256, 512, 1024}) {
     const int dyn = lds_per_workgroup - static_lds_bytes(wg);
-    for (int which = 0; which < 3; ++which) {
+    for (int which = 0; which < 6; ++which) {
       hipError_t e = hipFuncSetAttribute(instance(which, wg),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
       if (e != hipSuccess) return (int)e;
@@ -61,10 +64,11 @@ int prepare_kernels(int lds_per_workgroup) {
 
 int launch_sift(int persistent, int wg, int grid, const Batch& b, const SiftParams& P,
                 hipStream_t stream) {
-  void* fn = sift_kernel_ptr(persistent, wg);
+  const int timed = b.times != nullptr;
+  void* fn = sift_kernel_ptr(persistent, wg, timed);
   if (!fn) return (int)hipErrorInvalidValue;
   if (grid <= 0) return (int)hipSuccess;
-  const size_t lds = dyn_lds_bytes(P.tile_slots, P.max_n);
+  const size_t lds = dyn_lds_bytes(P.tile_slots, P.max_n, timed ? kSlotBytesTime : kSlotBytes);
   Batch bb = b;
   SiftParams pp = P;
   void* args[] = {&bb, &pp};
@@ -72,10 +76,11 @@ int launch_sift(int persistent, int wg, int grid, const Batch& b, const SiftPara
 }
 
 int launch_envelope(int wg, const EnvBatch& e, const SiftParams& P, hipStream_t stream) {
-  void* fn = envelope_kernel_ptr(wg);
+  const int timed = e.times != nullptr;
+  void* fn = envelope_kernel_ptr(wg, timed);
   if (!fn) return (int)hipErrorInvalidValue;
   if (e.B <= 0) return (int)hipSuccess;
-  const size_t lds = dyn_lds_bytes(P.tile_slots, P.max_n);
+  const size_t lds = dyn_lds_bytes(P.tile_slots, P.max_n, timed ? kSlotBytesTime : kSlotBytes);
   EnvBatch ee = e;
   SiftParams pp = P;
   void* args[] = {&ee, &pp};
@@ -87,8 +92,8 @@ int reset_queue(unsigned* q, hipStream_t stream) {
   return (int)hipGetLastError();
 }
 
-int max_resident_wgs(int persistent, int wg, size_t dyn_lds) {
-  void* fn = sift_kernel_ptr(persistent, wg);
+int max_resident_wgs(int persistent, int wg, size_t dyn_lds, int timed) {
+  void* fn = sift_kernel_ptr(persistent, wg, timed);
   if (!fn) return 0;
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, wg, dyn_lds) != hipSuccess) return 0;

@@ -8,6 +8,7 @@ namespace emdk {
 constexpr int kMaxNbsym = 4;      // mirrored points per side supported by the kernels
 constexpr int kMaxN = 16384;      // longest trace (256 blocks of 64 samples)
 constexpr int kSlotBytes = 28;    // bytes per knot slot: position int32, value, M, 1/h fp64
+constexpr int kSlotBytesTime = 32;  // the same with an fp64 time position
 constexpr int kWorkBlockBytes = 568;  // LDS bytes per 64-sample block of the work area
 
 struct SiftParams {
@@ -43,6 +44,9 @@ struct Batch {
   char* spill;                 // per-workgroup global knot tiles
   int64_t spill_stride;        // bytes per workgroup
   unsigned* queue;             // persistent-kernel queue head
+  const double* times;         // time-axis kernels: sample times (normalised), laid out
+                               // like values, or one shared row when times_shared
+  int times_shared;
 };
 
 // Envelope-only pass outputs (reference envelope_batch_kernel ABI, kernels.py:1100-1113).
@@ -62,26 +66,29 @@ struct EnvBatch {
   int mbuf;
   char* spill;
   int64_t spill_stride;
+  const double* times;  // as in Batch: [B,N] rows, or one shared [N] row when times_shared
+  int times_shared;
 };
 
 
-// Bytes of one workgroup's global knot tile of `slots` slots.
-int64_t tile_bytes(int slots);
+// Bytes of one workgroup's global knot tile of `slots` slots of slot_bytes each.
+int64_t tile_bytes(int slots, int slot_bytes = kSlotBytes);
 // Static LDS of the sift kernels for a workgroup size.
 int static_lds_bytes(int wg);
 // Dynamic LDS of a launch: the per-trace work area (IMF + block bookkeeping) for traces of
 // up to max_n samples, plus the LDS knot tile.
-size_t dyn_lds_bytes(int tile_slots, int max_n);
+size_t dyn_lds_bytes(int tile_slots, int max_n, int slot_bytes = kSlotBytes);
 
 // Launch one sift kernel instance.  persistent != 0 selects the queue kernel, whose grid
-// is `grid` workgroups; otherwise grid is the number of traces of the launch.  Returns a
-// hipError_t.
+// is `grid` workgroups; otherwise grid is the number of traces of the launch.  A batch with
+// times runs the time-axis instance (knot slots of kSlotBytesTime).  Returns a hipError_t.
 int launch_sift(int persistent, int wg, int grid, const Batch& b, const SiftParams& P,
                 hipStream_t stream);
 int launch_envelope(int wg, const EnvBatch& e, const SiftParams& P, hipStream_t stream);
 int reset_queue(unsigned* q, hipStream_t stream);
-// Resident workgroups per CU for a kernel instance at the given dynamic LDS.
-int max_resident_wgs(int persistent, int wg, size_t dyn_lds);
+// Resident workgroups per CU for a kernel instance at the given dynamic LDS (timed: the
+// time-axis instance).
+int max_resident_wgs(int persistent, int wg, size_t dyn_lds, int timed = 0);
 // Raise every kernel's dynamic-LDS limit to the device's per-workgroup LDS (once per handle).
 int prepare_kernels(int lds_per_workgroup);
 

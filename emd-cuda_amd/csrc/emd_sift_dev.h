@@ -56,21 +56,45 @@ constexpr int kWorkBytesPerBlock = 568;  // 64 fp64 + 4 fp64 + 2 u64 + 2 int per
 
 enum SplineKind { kConst = 0, kLinear = 1, kThreePoint = 2, kNotAKnot = 3 };
 
-struct Tile {
+// Knot and sample positions.  IndexAxis: PyEMD's simple mode, where a sample's position
+// is its index (int, exact).  TimeAxis: the caller's time axis, already normalised as the
+// reference does ((T - T[0]) / min(diff T), emd.py:83-88), read from global memory; the
+// reference places knots at T[idx] and mirrored knots at 2 T[sym] - T[idx]
+// (kernels.py:471-472, 528-552) and evaluates the envelopes at T[i] (kernels.py:703).
+struct IndexAxis {
+  using Pos = int;
+  static constexpr int kSlot = kSlotBytes;
+  __device__ __forceinline__ int at(int i) const { return i; }
+  // the position of sample i, where i may be one past the last sample (not used there)
+  __device__ __forceinline__ int at_upto(int i, int) const { return i; }
+};
+struct TimeAxis {
+  using Pos = double;
+  static constexpr int kSlot = kSlotBytesTime;
+  const double* t;  // this trace's N times
+  __device__ __forceinline__ double at(int i) const { return t[i]; }
+  __device__ __forceinline__ double at_upto(int i, int last) const { return t[min(i, last)]; }
+};
+
+template <class P>
+struct TileT {
   EMD_ARR(double) Y;  // knot values
   EMD_ARR(double) M;  // chord slope of the interval that starts at each knot slot,
                       // overwritten by the solve with the second derivative at the slot
   EMD_ARR(double) H;  // 1/h of the interval that starts at each knot slot
-  EMD_ARR(int) X;     // knot positions (sample index; mirrored knots may be <0 or >N-1)
+  EMD_ARR(P) X;       // knot positions (sample index, or time; mirrored knots may lie
+                      // before the first sample or after the last)
 };
+using Tile = TileT<int>;
 
-__device__ __forceinline__ Tile make_tile(double* base, int slots) {
-  Tile t;
+template <class P>
+__device__ __forceinline__ TileT<P> make_tile(double* base, int slots) {
+  TileT<P> t;
   t.Y = EMD_SPAN(double, base, slots, "tile.Y");
   t.M = EMD_SPAN(double, base + slots, slots, "tile.M");
   t.H = EMD_SPAN(double, base + 2 * slots, slots, "tile.H");
-  int* x = reinterpret_cast<int*>(base + 3 * slots);
-  t.X = EMD_SPAN(int, x, slots, "tile.X");
+  P* x = reinterpret_cast<P*>(base + 3 * slots);
+  t.X = EMD_SPAN(P, x, slots, "tile.X");
   return t;
 }
 
@@ -350,9 +374,11 @@ __device__ __forceinline__ void serial_extrema(const double* S, int N, bool want
 }
 
 // Blocks first, first + step, ... of the calling wave.
-__device__ __forceinline__ void compact_knots(const Work& wk, int N, const Tile& t, int nm,
-                                              int nn, int pad, uint64_t& bad, int first,
-                                              int step) {
+template <class AX>
+__device__ __forceinline__ void compact_knots(const Work& wk, int N,
+                                              const TileT<typename AX::Pos>& t, const AX& ax,
+                                              int nm, int nn, int pad, uint64_t& bad,
+                                              int first, int step) {
   const int lane = fresh_tid() & 63;
   const int nblk = ceil_div(N, 64);
   const int tb0 = pad, tb1 = 3 * pad + nm;
@@ -368,11 +394,11 @@ __device__ __forceinline__ void compact_knots(const Work& wk, int N, const Tile&
     const bool sm = ((bm >> lane) & 1ull) && km < nm;  // this lane stores a maximum
     const bool sn = ((bn >> lane) & 1ull) && kn < nn;  // or a minimum (max_extrema clamp)
     if (sm) {
-      t.X[tb0 + km] = i;
+      t.X[tb0 + km] = ax.at(i);
       t.Y[tb0 + km] = y;
     }
     if (sn) {
-      t.X[tb1 + kn] = i;
+      t.X[tb1 + kn] = ax.at(i);
       t.Y[tb1 + kn] = y;
     }
     bad |= (__ballot(sm) & fmask<kFOlt>(y)) | (__ballot(sn) & fmask<kFOgt>(y));
@@ -381,7 +407,6 @@ __device__ __forceinline__ void compact_knots(const Work& wk, int N, const Tile&
 
 template <int WG>
 __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared<WG>& sh,
-                                               const Tile& tl, const Tile& tg,
                                                const SiftParams& P) {
   constexpr int NW = WG / 64;
   const int tid = fresh_tid(), lane = tid & 63, w = uni(tid >> 6);
@@ -619,13 +644,19 @@ __device__ __forceinline__ double kd_val(const KDesc& d, const KnotEnds& c, int 
 }
 
 // Knot counts of one side after PyEMD's removal of duplicate adjacent positions.
-__device__ __forceinline__ void side_counts(const KnotEnds& c, int side, int n, int tb,
-                                            const KDesc& L, int lsym, const KDesc& R, int rsym,
-                                            int& nl, int& rs, int& k0, int& nk, int& kind) {
+template <class AX>
+__device__ __forceinline__ void side_counts(const KnotEnds& c, const AX& ax, int side, int n,
+                                            int tb, const KDesc& L, int lsym, const KDesc& R,
+                                            int rsym, int& nl, int& rs, int& k0, int& nk,
+                                            int& kind) {
   const int ln = kd_len(L), rn = kd_len(R);
   nl = ln;
-  if (ln > 0 && 2 * lsym - kd_idx(L, c, side, ln - 1) == ends_x(c, side, 0)) --nl;
-  rs = (rn > 0 && 2 * rsym - kd_idx(R, c, side, 0) == ends_x(c, side, n - 1)) ? 1 : 0;
+  if (ln > 0 && 2 * ax.at(lsym) - ax.at(kd_idx(L, c, side, ln - 1)) == ax.at(ends_x(c, side, 0)))
+    --nl;
+  rs = (rn > 0 &&
+        2 * ax.at(rsym) - ax.at(kd_idx(R, c, side, 0)) == ax.at(ends_x(c, side, n - 1)))
+           ? 1
+           : 0;
   k0 = tb - nl;
   nk = nl + n + (rn - rs);
   kind = nk >= 4 ? kNotAKnot : nk == 3 ? kThreePoint : nk == 2 ? kLinear : kConst;
@@ -637,15 +668,17 @@ __device__ __forceinline__ void side_counts(const KnotEnds& c, int side, int n, 
 // 2*lsym - idx; a left knot equal in position to the first true knot is the same sample
 // (same value) and is dropped, which is what PyEMD's delete of equal adjacent positions
 // does; likewise on the right.  Right knots go after the true ones.
-__device__ __forceinline__ void finish_lists(const Tile& t, const KnotEnds& c, int tb0, int tb1,
-                                             int nm, int nn, const KDesc& lmax, const KDesc& lmin,
+template <class AX>
+__device__ __forceinline__ void finish_lists(const TileT<typename AX::Pos>& t, const AX& ax,
+                                             const KnotEnds& c, int tb0, int tb1, int nm, int nn,
+                                             const KDesc& lmax, const KDesc& lmin,
                                              const KDesc& rmax, const KDesc& rmin, int lsym,
                                              int rsym, int* k0, int* nk, int* kind, int& bad) {
   constexpr int kR = kMaxNbsym + 1;
   const int lane = fresh_tid() & 63;
   int nl[2], rs[2];
-  side_counts(c, 0, nm, tb0, lmax, lsym, rmax, rsym, nl[0], rs[0], k0[0], nk[0], kind[0]);
-  side_counts(c, 1, nn, tb1, lmin, lsym, rmin, rsym, nl[1], rs[1], k0[1], nk[1], kind[1]);
+  side_counts(c, ax, 0, nm, tb0, lmax, lsym, rmax, rsym, nl[0], rs[0], k0[0], nk[0], kind[0]);
+  side_counts(c, ax, 1, nn, tb1, lmin, lsym, rmin, rsym, nl[1], rs[1], k0[1], nk[1], kind[1]);
   // list of this lane: 0 left maxima, 1 right maxima, 2 left minima, 3 right minima
   const int list = lane / kR, e = lane - list * kR;
   const int side = list >> 1;
@@ -670,10 +703,10 @@ __device__ __forceinline__ void finish_lists(const Tile& t, const KnotEnds& c, i
     const int n = side ? nn : nm, tb = side ? tb1 : tb0;
     const int nls = side ? nl[1] : nl[0], rss = side ? rs[1] : rs[0];
     if (!right && e < nls) {
-      t.X[tb - nls + e] = 2 * lsym - idx;
+      t.X[tb - nls + e] = 2 * ax.at(lsym) - ax.at(idx);
       t.Y[tb - nls + e] = val;
     } else if (right && e >= rss) {
-      t.X[tb + n + e - rss] = 2 * rsym - idx;
+      t.X[tb + n + e - rss] = 2 * ax.at(rsym) - ax.at(idx);
       t.Y[tb + n + e - rss] = val;
     }
   }
@@ -681,7 +714,8 @@ __device__ __forceinline__ void finish_lists(const Tile& t, const KnotEnds& c, i
 
 // PyEMD cubic_spline_3pts: the natural cubic spline through the three knots from slot q0,
 // in slope form, solved in closed form (coefficients a1, b1, a2, b2 of its two intervals).
-__device__ __forceinline__ void three_point_coeffs(const Tile& t, int q0, double* p3) {
+template <class P>
+__device__ __forceinline__ void three_point_coeffs(const TileT<P>& t, int q0, double* p3) {
   const double x0 = t.X[q0], x1 = t.X[q0 + 1], x2 = t.X[q0 + 2];
   const double y0 = t.Y[q0], y1 = t.Y[q0 + 1], y2 = t.Y[q0 + 2];
   const double h1 = x1 - x0, h2 = x2 - x1;
@@ -699,11 +733,15 @@ __device__ __forceinline__ void three_point_coeffs(const Tile& t, int q0, double
 }
 
 // Boundary mirroring, run wave-uniformly by one wave (its results and knot writes are the
-// same in every lane; lane 0 writes).  PyEMD _prepare_points_simple with T = arange(N);
-// the reference's mirror_one_side (kernels.py:360-558) follows the same branch table.
-__device__ __forceinline__ void mirror(const Tile& t, const Work& wk, int N, int nm, int nn,
-                                       int nb, double S0, double SN, int* k0, int* nk,
-                                       int* kind, int& bad, int* xf, int* xl) {
+// same in every lane; lane 0 writes).  PyEMD _prepare_points_simple (positions from the
+// axis: T = arange(N) in PyEMD's simple mode); the reference's mirror_one_side
+// (kernels.py:360-558) follows the same branch table.  xf, xl: sample indices of the first
+// and last true knot of each side.
+template <class AX>
+__device__ __forceinline__ void mirror(const TileT<typename AX::Pos>& t, const AX& ax,
+                                       const Work& wk, int N, int nm, int nn, int nb, double S0,
+                                       double SN, int* k0, int* nk, int* kind, int& bad, int* xf,
+                                       int* xl) {
   const int tb0 = nb, tb1 = 3 * nb + nm;
   const KnotEnds c = ends_from_masks(wk, ceil_div(N, 64), nm, nn, nb + 1, N, S0, SN);
   const int Xm0 = ends_x(c, 0, 0), XmL = ends_x(c, 0, nm - 1);
@@ -770,18 +808,20 @@ __device__ __forceinline__ void mirror(const Tile& t, const Work& wk, int N, int
   if (kd_len(rmax) == 0) rmax = kd(0, nm, 0);
 
   // mirrored points must reach the signal edges; otherwise mirror about the edge
-  if (2 * lsym - kd_idx(lmin, c, 1, 0) > 0 || 2 * lsym - kd_idx(lmax, c, 0, 0) > 0) {
+  if (2 * ax.at(lsym) - ax.at(kd_idx(lmin, c, 1, 0)) > ax.at(0) ||
+      2 * ax.at(lsym) - ax.at(kd_idx(lmax, c, 0, 0)) > ax.at(0)) {
     if (lsym == Xm0) lmax = kd(0, min(nm, nb), 1);
     else lmin = kd(0, min(nn, nb), 1);
     lsym = 0;
   }
-  if (2 * rsym - kd_idx(rmin, c, 1, kd_len(rmin) - 1) < N - 1 ||
-      2 * rsym - kd_idx(rmax, c, 0, kd_len(rmax) - 1) < N - 1) {
+  if (2 * ax.at(rsym) - ax.at(kd_idx(rmin, c, 1, kd_len(rmin) - 1)) < ax.at(N - 1) ||
+      2 * ax.at(rsym) - ax.at(kd_idx(rmax, c, 0, kd_len(rmax) - 1)) < ax.at(N - 1)) {
     if (rsym == XmL) rmax = kd(max(nm - nb, 0), nm, 1);
     else rmin = kd(max(nn - nb, 0), nn, 1);
     rsym = N - 1;
   }
-  finish_lists(t, c, tb0, tb1, nm, nn, lmax, lmin, rmax, rmin, lsym, rsym, k0, nk, kind, bad);
+  finish_lists(t, ax, c, tb0, tb1, nm, nn, lmax, lmin, rmax, rmin, lsym, rsym, k0, nk, kind,
+               bad);
 }
 
 // =======================================================================================
@@ -799,7 +839,8 @@ __device__ __forceinline__ void mirror(const Tile& t, const Work& wk, int N, int
 // =======================================================================================
 // Per interval k (knots k, k+1) of a not-a-knot side: H[k] = 1/h and, until the solve
 // overwrites it, M[k] = the chord slope (y_{k+1} - y_k) / h as (y_{k+1} - y_k) * (1/h).
-__device__ __forceinline__ void interval_setup(const Tile& t, int k) {
+template <class P>
+__device__ __forceinline__ void interval_setup(const TileT<P>& t, int k) {
   const double ih = recip(double(t.X[k + 1] - t.X[k]));
   t.H[k] = ih;
   t.M[k] = (t.Y[k + 1] - t.Y[k]) * ih;
@@ -807,7 +848,8 @@ __device__ __forceinline__ void interval_setup(const Tile& t, int k) {
 
 // Row j of the system of a side whose knots start at slot k0: a x_{j-1} + b x_j + c x_{j+1} = d,
 // from the chord slopes sl, sr of the intervals left and right of knot k0+j+1.
-__device__ __forceinline__ void nak_row_sl(const Tile& t, int k0, int m, int j, double sl,
+template <class P>
+__device__ __forceinline__ void nak_row_sl(const TileT<P>& t, int k0, int m, int j, double sl,
                                            double sr, double& a, double& b, double& c,
                                            double& d) {
   const int i = k0 + j + 1;
@@ -833,8 +875,9 @@ __device__ __forceinline__ void nak_row_sl(const Tile& t, int k0, int m, int j, 
 }
 
 // The same row with the chord slopes the interval setup left in M.
-__device__ __forceinline__ void nak_row(const Tile& t, int k0, int m, int j, double& a, double& b,
-                                        double& c, double& d) {
+template <class P>
+__device__ __forceinline__ void nak_row(const TileT<P>& t, int k0, int m, int j, double& a,
+                                        double& b, double& c, double& d) {
   const int i = k0 + j + 1;
   nak_row_sl(t, k0, m, j, t.M[i - 1], t.M[i], a, b, c, d);
 }
@@ -850,19 +893,20 @@ struct EndRows {
   double ea, eb, ec, ed;  // end row:   ea x_s + eb x_e + ec x_{e+1} = ed
 };
 
-__device__ __forceinline__ EndRows chunk_ends(const Tile& t, int k0, int m, int s, int e) {
+template <class P>
+__device__ __forceinline__ EndRows chunk_ends(const TileT<P>& t, int k0, int m, int s, int e) {
   const int len = e - s - 1;
   const double sle = t.M[k0 + e];
-  int xm = t.X[k0 + e], xr = t.X[k0 + e + 1];
+  P xm = t.X[k0 + e], xr = t.X[k0 + e + 1];
   double sr = sle;
   double p = 0.0, pq = 0.0, pr = 0.0, Q = 1.0, su = 0.0, sw = 0.0;
   // each row's left knot and slope are loaded one row ahead (the last such load is of the
   // start row's, always a slot of this chunk)
-  int xl_n = t.X[k0 + e - 1];
+  P xl_n = t.X[k0 + e - 1];
   double sl_n = t.M[k0 + e - 1];
   for (int q = 0; q < len; ++q) {
     const int j = e - 1 - q;  // knots k0+j .. k0+j+2
-    const int xl = xl_n;
+    const P xl = xl_n;
     const double sl = sl_n;
     xl_n = t.X[k0 + j - 1];
     sl_n = t.M[k0 + j - 1];
@@ -903,20 +947,21 @@ __device__ __forceinline__ EndRows chunk_ends(const Tile& t, int k0, int m, int 
 // parked (q_j, p_j), x_j = p_j - q_j x_{j-1} - r_j x_e, with r_j rebuilt from r_{s+1} by
 // r_{j+1} = -r_j (b_j - c_j q_{j+1}) / c_j; x_j replaces p_j in M[k0+j+1], 1/h of interval
 // k0+j goes back into H[k0+j], and x_s, x_e are stored last.
-__device__ __forceinline__ void chunk_interior(const Tile& t, int k0, int s, int e, double xs,
-                                               double xe) {
+template <class P>
+__device__ __forceinline__ void chunk_interior(const TileT<P>& t, int k0, int s, int e,
+                                               double xs, double xe) {
   const int len = e - s - 1;
   if (len > 0) {
     double x = xs, r = t.M[k0 + s + 1], qj = t.H[k0 + s + 1];
-    int xl = t.X[k0 + s + 1], xm = t.X[k0 + s + 2];
+    P xl = t.X[k0 + s + 1], xm = t.X[k0 + s + 2];
     double ihl = recip(double(xm - xl));
     // each row's right knot, parked p and next q are loaded one row ahead (the last such
     // loads reach knot k0+e+2 and slot k0+e+1, still this side's)
-    int xr_n = t.X[k0 + s + 3];
+    P xr_n = t.X[k0 + s + 3];
     double p_n = t.M[k0 + s + 2], q_n = t.H[k0 + s + 2];
     for (int q = 0; q < len; ++q) {
       const int j = s + 1 + q;  // knots k0+j .. k0+j+2
-      const int xr = xr_n;
+      const P xr = xr_n;
       const double pj = p_n, qn = q_n;
       xr_n = t.X[k0 + j + 3];
       p_n = t.M[k0 + j + 2];
@@ -941,7 +986,8 @@ __device__ __forceinline__ void chunk_interior(const Tile& t, int k0, int s, int
 
 // Solve one side's system (knots from slot k0, m rows) with the calling wave; every lane
 // of the wave must call it.  Writes M[k0+1 .. k0+m].
-__device__ __forceinline__ void spline_system_wave(const Tile& t, int side, int k0, int m,
+template <class P>
+__device__ __forceinline__ void spline_system_wave(const TileT<P>& t, int side, int k0, int m,
                                                    int lane) {
   const int K = min(64, m / 2);  // chunks of >= 2 rows
   // m / K and m % K without a division: K = 64 for m >= 128; otherwise K = m / 2, which
@@ -1021,7 +1067,8 @@ __device__ __forceinline__ void spline_system_wave(const Tile& t, int side, int 
 // Not-a-knot solve of one side by the calling wave (every lane calls): interval setup of
 // the intervals that touch mirrored knots (the true-knot intervals were set up while the
 // mirror ran), the solve, then M_0 and M_{n-1} from the not-a-knot relations.
-__device__ __forceinline__ void solve_side(const Tile& t, int side, int k0, int nk, int tb,
+template <class P>
+__device__ __forceinline__ void solve_side(const TileT<P>& t, int side, int k0, int nk, int tb,
                                            int n, int lane) {
   const int nl = tb - k0;  // left mirrored knots
   const int nmir = nk - n;  // intervals with a mirrored end: nl on the left, the rest right
@@ -1042,8 +1089,8 @@ __device__ __forceinline__ void solve_side(const Tile& t, int side, int k0, int 
   }
 }
 
-template <int WG>
-__device__ __forceinline__ void spline_solve(const Tile& t, const SiftShared<WG>& sh,
+template <int WG, class P>
+__device__ __forceinline__ void spline_solve(const TileT<P>& t, const SiftShared<WG>& sh,
                                              const DetectResult& D, int nb) {
   const int tid = fresh_tid(), lane = tid & 63, wv = uni(tid >> 6);
   const int mtot = uni(sh.mtot);
@@ -1066,9 +1113,9 @@ __device__ __forceinline__ void spline_solve(const Tile& t, const SiftShared<WG>
 
 // Envelope value at a sample for the closed-form spline kinds (3 knots: PyEMD's natural
 // cubic; 2 knots: straight line; 1 knot: constant).
-__device__ __forceinline__ double eval_small(const Tile& t, int k0, int kind, const double* p3,
-                                             int pos) {
-  const double tt = double(pos);
+template <class P>
+__device__ __forceinline__ double eval_small(const TileT<P>& t, int k0, int kind,
+                                             const double* p3, double tt) {
   const double X0 = t.X[k0], Y0 = t.Y[k0];
   if (kind == kConst) return Y0;
   const double X1 = t.X[k0 + 1], Y1 = t.Y[k0 + 1];
@@ -1087,31 +1134,33 @@ __device__ __forceinline__ double eval_small(const Tile& t, int k0, int kind, co
 // Interval of not-a-knot side `side` holding sample i (block blk, lane) from the masks of
 // the detection that produced the knots: true extrema at or before i plus the left knots,
 // corrected for left/right mirrored knots that straddle i (at most nbsym steps).
-template <int WG>
-__device__ __forceinline__ int interval_of(const Tile& t, const SiftShared<WG>& sh, int side,
-                                           const Work& wk, int blk, int lane, int i, int tb) {
+template <int WG, class P>
+__device__ __forceinline__ int interval_of(const TileT<P>& t, const SiftShared<WG>& sh, int side,
+                                           const Work& wk, int blk, int lane, P pi, int tb) {
   const int k0 = sh.k0[side], nk = sh.nk[side];
   const uint64_t m = side == 0 ? wk.mmax[blk] : wk.mmin[blk];
   const int pre = side == 0 ? wk.pmax[blk] : wk.pmin[blk];
   const int le = pre + __popcll(m & (lanemask_lt(lane) | (uint64_t(1) << lane)));
   int j = (tb - k0) + le - 1;
   j = max(0, min(j, nk - 2));
-  while (j < nk - 2 && t.X[k0 + j + 1] <= i) ++j;
-  while (j > 0 && t.X[k0 + j] > i) --j;
+  while (j < nk - 2 && t.X[k0 + j + 1] <= pi) ++j;
+  while (j > 0 && t.X[k0 + j] > pi) --j;
   return j;
 }
 
-template <int WG>
-__device__ __forceinline__ double envelope_at(const Tile& t, const SiftShared<WG>& sh, int side,
-                                              const Work& wk, int blk, int lane, int i, int tb) {
+// Envelope of side `side` at sample i, whose position is pi.
+template <int WG, class P>
+__device__ __forceinline__ double envelope_at(const TileT<P>& t, const SiftShared<WG>& sh,
+                                              int side, const Work& wk, int blk, int lane, P pi,
+                                              int tb) {
   const int kind = sh.kind[side];
   const int k0 = sh.k0[side];
-  if (kind != kNotAKnot) return eval_small(t, k0, kind, sh.p3[side], i);
-  const int j = interval_of(t, sh, side, wk, blk, lane, i, tb) + k0;
-  const int x0 = t.X[j], x1 = t.X[j + 1];
+  if (kind != kNotAKnot) return eval_small(t, k0, kind, sh.p3[side], double(pi));
+  const int j = interval_of(t, sh, side, wk, blk, lane, pi, tb) + k0;
+  const P x0 = t.X[j], x1 = t.X[j + 1];
   const double h = double(x1 - x0);
-  const double A = double(x1 - i) * t.H[j];
-  const double B = double(i - x0) * t.H[j];
+  const double A = double(x1 - pi) * t.H[j];
+  const double B = double(pi - x0) * t.H[j];
   return A * t.Y[j] + B * t.Y[j + 1] +
          ((A * A * A - A) * t.M[j] + (B * B * B - B) * t.M[j + 1]) * (h * h * kSixth);
 }
@@ -1205,10 +1254,12 @@ __device__ __forceinline__ SideEval side_eval(const SiftShared<WG>& sh, int side
 // b).  The interval comes from the extrema masks of the detection that produced the knots:
 // true extrema at or before the sample plus the left knots.  Only samples before the
 // first or after the last true knot (where mirrored knots may straddle them) walk to the
-// enclosing interval; for every other sample the walk would not move.
-template <int SIDE>
-__device__ __forceinline__ void envelope_pair(const Tile& t, const SideEval& s, const Work& wk,
-                                              int b, int p, int i, double& e0, double& e1) {
+// enclosing interval; for every other sample the walk would not move.  pi0, pi1: the
+// positions of samples i and i+1.
+template <int SIDE, class P>
+__device__ __forceinline__ void envelope_pair(const TileT<P>& t, const SideEval& s,
+                                              const Work& wk, int b, int p, int i, P pi0, P pi1,
+                                              double& e0, double& e1) {
   const uint64_t m = SIDE == 0 ? wk.mmax[b] : wk.mmin[b];
   const int pre = SIDE == 0 ? wk.pmax[b] : wk.pmin[b];
   const uint64_t le_mask = (uint64_t(2) << p) - 1;  // bits 0..p (p is even, so <= 62)
@@ -1217,15 +1268,16 @@ __device__ __forceinline__ void envelope_pair(const Tile& t, const SideEval& s, 
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     const int ii = i + q;
+    const P pi = q ? pi1 : pi0;
     int j = s.nl + (q ? le1 : le0) - 1;
     if (!(ii >= s.xf && ii < s.xl)) {
       j = max(0, min(j, s.nk2));
-      while (j < s.nk2 && t.X[s.k0 + j + 1] <= ii) ++j;
-      while (j > 0 && t.X[s.k0 + j] > ii) --j;
+      while (j < s.nk2 && t.X[s.k0 + j + 1] <= pi) ++j;
+      while (j > 0 && t.X[s.k0 + j] > pi) --j;
     }
     const int jj = s.k0 + j;
-    const int xa = t.X[jj], xb = t.X[jj + 1];
-    const double B = double(ii - xa) * t.H[jj];
+    const P xa = t.X[jj], xb = t.X[jj + 1];
+    const double B = double(pi - xa) * t.H[jj];
     const double A = 1.0 - B;
     const double hh = double(xb - xa);
     // A y0 + B y1 + h^2/6 ((A^3 - A) M0 + (B^3 - B) M1), with fused multiply-adds
@@ -1240,28 +1292,31 @@ __device__ __forceinline__ void envelope_pair(const Tile& t, const SideEval& s, 
 // One 128-sample block c of the envelope pass: both envelopes at samples i and i+1 of this
 // lane, the IMF update, and the lane's pair sums of the three stop statistics (in sq).
 // NAK: both sides are known to be not-a-knot.
-template <int WG, bool NAK>
-__device__ __forceinline__ void envelope_block(const Work& wk, const Tile& t,
+template <int WG, bool NAK, class AX>
+__device__ __forceinline__ void envelope_block(const Work& wk,
+                                               const TileT<typename AX::Pos>& t, const AX& ax,
                                                const SiftShared<WG>& sh, int N, int c, int lane,
                                                const SideEval& s0, const SideEval& s1,
                                                int kind0, int kind1, double* sq, double& nmax,
                                                double& nmin) {
+  using P = typename AX::Pos;
   const int i = c * 128 + 2 * lane;
   sq[0] = sq[1] = sq[2] = 0.0;
   if (i < N) {
     const int b = 2 * c + (lane >> 5), p = (2 * lane) & 63;
+    const P pi0 = ax.at(i), pi1 = ax.at_upto(i + 1, N - 1);
     double emax0, emax1, emin0, emin1;
     if (NAK || kind0 == kNotAKnot) {
-      envelope_pair<0>(t, s0, wk, b, p, i, emax0, emax1);
+      envelope_pair<0>(t, s0, wk, b, p, i, pi0, pi1, emax0, emax1);
     } else {
-      emax0 = eval_small(t, sh.k0[0], kind0, sh.p3[0], i);
-      emax1 = eval_small(t, sh.k0[0], kind0, sh.p3[0], i + 1);
+      emax0 = eval_small(t, sh.k0[0], kind0, sh.p3[0], double(pi0));
+      emax1 = eval_small(t, sh.k0[0], kind0, sh.p3[0], double(pi1));
     }
     if (NAK || kind1 == kNotAKnot) {
-      envelope_pair<1>(t, s1, wk, b, p, i, emin0, emin1);
+      envelope_pair<1>(t, s1, wk, b, p, i, pi0, pi1, emin0, emin1);
     } else {
-      emin0 = eval_small(t, sh.k0[1], kind1, sh.p3[1], i);
-      emin1 = eval_small(t, sh.k0[1], kind1, sh.p3[1], i + 1);
+      emin0 = eval_small(t, sh.k0[1], kind1, sh.p3[1], double(pi0));
+      emin1 = eval_small(t, sh.k0[1], kind1, sh.p3[1], double(pi1));
     }
     // i + 1 < 64 * nbcap always (i is even and < N), so the pair read stays in the work area
     const double old0 = wk.imf[i], old1 = wk.imf[i + 1];
@@ -1303,9 +1358,10 @@ __device__ __forceinline__ int sub_blocks(int nblk2) { return min(4, max(1, nblk
 // Sift n's imf_old is the IMF that sift n-1 left, bit for bit (the detection in between
 // does not touch it), so its sum of squares and its range come from sh.carry[(n-1)&1]
 // instead of being recomputed; the IMF's start pass fills carry[0] for n == 1.
-template <int WG>
-__device__ __forceinline__ SiftStats envelope_update(const Work& wk, const Tile& t,
-                                                     SiftShared<WG>& sh, int N,
+template <int WG, class AX>
+__device__ __forceinline__ SiftStats envelope_update(const Work& wk,
+                                                     const TileT<typename AX::Pos>& t,
+                                                     const AX& ax, SiftShared<WG>& sh, int N,
                                                      const DetectResult& D, int nb, int n) {
   constexpr int NW = WG / 64;
   const int tid = fresh_tid(), lane = tid & 63, w = uni(tid >> 6);
@@ -1324,9 +1380,11 @@ __device__ __forceinline__ SiftStats envelope_update(const Work& wk, const Tile&
       double v[3];
       // the common case (both envelopes not-a-knot) gets a loop without the small kinds
       if (nak)
-        envelope_block<WG, true>(wk, t, sh, N, c, lane, s0, s1, kind0, kind1, v, nmax, nmin);
+        envelope_block<WG, true>(wk, t, ax, sh, N, c, lane, s0, s1, kind0, kind1, v, nmax,
+                                 nmin);
       else
-        envelope_block<WG, false>(wk, t, sh, N, c, lane, s0, s1, kind0, kind1, v, nmax, nmin);
+        envelope_block<WG, false>(wk, t, ax, sh, N, c, lane, s0, s1, kind0, kind1, v, nmax,
+                                  nmin);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         if (q == 0) a[k] = v[k];
@@ -1389,17 +1447,18 @@ __device__ __forceinline__ bool check_imf(const SiftStats& s, int bad) {
 // sift and the envelope-only kernel).  Wave 0 mirrors (its end knots come from the extrema
 // masks) while the other waves compact the true extrema into the tile; after a barrier every
 // wave sets up the intervals between true knots, and waves 0 and 1 solve.
-template <int WG>
-__device__ __forceinline__ void build_envelopes(const Tile& t, SiftShared<WG>& sh,
-                                                const DetectResult& D, int N,
-                                                const SiftParams& P, const Work& wk) {
+template <int WG, class AX>
+__device__ __forceinline__ void build_envelopes(const TileT<typename AX::Pos>& t, const AX& ax,
+                                                SiftShared<WG>& sh, const DetectResult& D,
+                                                int N, const SiftParams& P, const Work& wk) {
   constexpr int NW = WG / 64;
   const int tid = fresh_tid(), lane = tid & 63, w = uni(tid >> 6);
   if (w == 0) {
     __builtin_amdgcn_s_setprio(2);  // the mirror is the longest single-wave step
     int bad = 0;
     int k0[2], nk[2], kind[2], xf[2], xl[2];
-    mirror(t, wk, N, D.nm, D.nn, P.nbsym, wk.imf[0], wk.imf[N - 1], k0, nk, kind, bad, xf, xl);
+    mirror(t, ax, wk, N, D.nm, D.nn, P.nbsym, wk.imf[0], wk.imf[N - 1], k0, nk, kind, bad, xf,
+           xl);
     if (tid == 0) {
       sh.bad_m = bad;
       for (int q = 0; q < 2; ++q) {
@@ -1419,7 +1478,8 @@ __device__ __forceinline__ void build_envelopes(const Tile& t, SiftShared<WG>& s
   // disjoint); wave 0 itself when it is the only one
   if (NW == 1 || w > 0) {
     uint64_t bad = 0;
-    compact_knots(wk, N, t, D.nm, D.nn, P.nbsym, bad, NW == 1 ? 0 : w - 1, NW == 1 ? 1 : NW - 1);
+    compact_knots(wk, N, t, ax, D.nm, D.nn, P.nbsym, bad, NW == 1 ? 0 : w - 1,
+                  NW == 1 ? 1 : NW - 1);
     if (lane == 0) sh.bad_w[w] = bad != 0ull ? 1 : 0;
   }
   __syncthreads();
@@ -1447,6 +1507,7 @@ __device__ __forceinline__ void build_envelopes(const Tile& t, SiftShared<WG>& s
 // =======================================================================================
 struct SignalIO {
   const double* S;
+  const double* T;  // the trace's times (time-axis kernels) or null
   int N;
   double* out;      // row k at out + k*N
   int* n_imfs;
@@ -1523,10 +1584,12 @@ __device__ __forceinline__ void start_imf(const SignalIO& io, SiftShared<WG>& sh
   }
 }
 
-// tl: the launch's LDS knot tile; tg: this workgroup's global tile (holds any trace).
-template <int WG>
-__device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh,
-                                          const Work& wk, const Tile& tl, const Tile& tg,
+// tl: the launch's LDS knot tile; tg: this workgroup's global tile (holds any trace); ax:
+// the positions of this trace's samples.
+template <int WG, class AX>
+__device__ __forceinline__ void decompose(const SignalIO& io, const AX& ax, SiftShared<WG>& sh,
+                                          const Work& wk, const TileT<typename AX::Pos>& tl,
+                                          const TileT<typename AX::Pos>& tg,
                                           const SiftParams& P) {
   constexpr int NW = WG / 64;
   const int tid = fresh_tid(), lane = tid & 63, w = uni(tid >> 6);
@@ -1565,7 +1628,7 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
 
   while (!finished && imf_count < P.max_imfs - 1) {
     if (imf_count == 0) start_imf<WG>(io, sh, wk, acc, false, false);
-    DetectResult D = detect<WG>(wk, N, sh, tl, tg, P);
+    DetectResult D = detect<WG>(wk, N, sh, P);
     overflow |= D.ovf;
     peak = max(peak, max(D.tm, D.tn));
     int n = 0;
@@ -1584,12 +1647,12 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
       if (extNo > 2 && D.nm >= 1 && D.nn >= 1) {
         SiftStats st;
         if (D.in_lds) {
-          build_envelopes<WG>(tl, sh, D, N, P, wk);
-          st = envelope_update<WG>(wk, tl, sh, N, D, nb, n);
+          build_envelopes<WG>(tl, ax, sh, D, N, P, wk);
+          st = envelope_update<WG>(wk, tl, ax, sh, N, D, nb, n);
         } else {
           flags |= 2;
-          build_envelopes<WG>(tg, sh, D, N, P, wk);
-          st = envelope_update<WG>(wk, tg, sh, N, D, nb, n);
+          build_envelopes<WG>(tg, ax, sh, D, N, P, wk);
+          st = envelope_update<WG>(wk, tg, ax, sh, N, D, nb, n);
         }
         // No barrier before the next detection: what this sift's last steps read (block
         // sums, sh.red, sh.carry, sh.bad, sh.nk) is written next only after the
@@ -1598,7 +1661,7 @@ __device__ __forceinline__ void decompose(const SignalIO& io, SiftShared<WG>& sh
         const int bad = uni(sh.bad);
         ++sifts;
         knot_sum += uni(sh.nk[0] + sh.nk[1]);
-        D = detect<WG>(wk, N, sh, tl, tg, P);
+        D = detect<WG>(wk, N, sh, P);
         overflow |= D.ovf;
         peak = max(peak, max(D.tm, D.tn));
         const int extNo2 = D.nm + D.nn;
@@ -1764,6 +1827,7 @@ __device__ __forceinline__ SignalIO signal_io(const Batch& b, int s, int max_imf
     off = (int64_t)s * n;
   }
   io.S = b.values + off;
+  io.T = b.times ? b.times + (b.times_shared ? 0 : off) : nullptr;
   io.N = (int)n;
   // default output layout: trace s's max_imfs rows start at (input offset) x max_imfs
   io.out = b.imfs + (b.out_offsets ? b.out_offsets[s] : off * max_imfs);
@@ -1778,39 +1842,52 @@ __device__ __forceinline__ SignalIO signal_io(const Batch& b, int s, int max_imf
 
 // Memory of a workgroup: the LDS work area for traces up to P.max_n samples, the LDS knot
 // tile after it, and the workgroup's global knot tile (P.spill_slots slots).
+template <class P>
 struct WgMem {
   Work wk;
-  Tile tl, tg;
+  TileT<P> tl, tg;
 };
 
-__device__ __forceinline__ WgMem workgroup_mem(double* lds, char* spill, int64_t stride,
-                                               const SiftParams& P) {
-  WgMem m;
-  const int nbcap = (P.max_n + 63) / 64;
+template <class P>
+__device__ __forceinline__ WgMem<P> workgroup_mem(double* lds, char* spill, int64_t stride,
+                                                  const SiftParams& Pr) {
+  WgMem<P> m;
+  const int nbcap = (Pr.max_n + 63) / 64;
   m.wk = make_work(lds, nbcap);
-  m.tl = make_tile(lds + (int64_t)nbcap * (kWorkBytesPerBlock / 8), P.tile_slots);
+  m.tl = make_tile<P>(lds + (int64_t)nbcap * (kWorkBytesPerBlock / 8), Pr.tile_slots);
   char* region = spill + (int64_t)blockIdx.x * stride;
-  m.tg = make_tile(reinterpret_cast<double*>(region), P.spill_slots);
+  m.tg = make_tile<P>(reinterpret_cast<double*>(region), Pr.spill_slots);
   return m;
 }
 
+// The positions of trace io's samples.
+__device__ __forceinline__ IndexAxis make_axis(IndexAxis, const SignalIO&) { return {}; }
+__device__ __forceinline__ TimeAxis make_axis(TimeAxis, const SignalIO& io) {
+  TimeAxis a;
+  a.t = io.T;
+  return a;
+}
+
 // One workgroup per trace (grid = traces of this launch).  The reference launch model.
-template <int WG>
+template <int WG, class AX>
 __device__ __forceinline__ void plain_body(const Batch& b, const SiftParams& P,
                                            SiftShared<WG>& sh, double* dyn_lds) {
-  const WgMem m = workgroup_mem(dyn_lds, b.spill, b.spill_stride, P);
+  const WgMem<typename AX::Pos> m =
+      workgroup_mem<typename AX::Pos>(dyn_lds, b.spill, b.spill_stride, P);
   const int q = b.base + (int)blockIdx.x;
   const int s = b.order ? b.order[q] : q;
   if ((unsigned)s >= (unsigned)b.total) return;  // a queue entry naming no trace
-  decompose<WG>(signal_io(b, s, P.max_imfs), sh, m.wk, m.tl, m.tg, P);
+  const SignalIO io = signal_io(b, s, P.max_imfs);
+  decompose<WG>(io, make_axis(AX{}, io), sh, m.wk, m.tl, m.tg, P);
 }
 
 // Persistent workgroups pull queue positions from an atomic counter until the b.B queue
 // entries are drained.
-template <int WG>
+template <int WG, class AX>
 __device__ __forceinline__ void persistent_body(const Batch& b, const SiftParams& P,
                                                 SiftShared<WG>& sh, double* dyn_lds) {
-  const WgMem m = workgroup_mem(dyn_lds, b.spill, b.spill_stride, P);
+  const WgMem<typename AX::Pos> m =
+      workgroup_mem<typename AX::Pos>(dyn_lds, b.spill, b.spill_stride, P);
   while (true) {
     if (threadIdx.x == 0) sh.queue_idx = (int)atomicAdd(b.queue, 1u);
     __syncthreads();
@@ -1820,38 +1897,43 @@ __device__ __forceinline__ void persistent_body(const Batch& b, const SiftParams
     const int q = b.base + idx;
     const int s = b.order ? b.order[q] : q;
     if ((unsigned)s >= (unsigned)b.total) continue;  // a queue entry naming no trace
-    decompose<WG>(signal_io(b, s, P.max_imfs), sh, m.wk, m.tl, m.tg, P);
+    const SignalIO io = signal_io(b, s, P.max_imfs);
+    decompose<WG>(io, make_axis(AX{}, io), sh, m.wk, m.tl, m.tg, P);
   }
 }
 
-template <int WG>
+template <int WG, class AX = IndexAxis>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void emd_plain_kernel(Batch b, SiftParams P) {
   extern __shared__ double dyn_lds[];
   __shared__ SiftShared<WG> sh;
-  plain_body<WG>(b, P, sh, dyn_lds);
+  plain_body<WG, AX>(b, P, sh, dyn_lds);
 }
 
-template <int WG>
+template <int WG, class AX = IndexAxis>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void emd_persistent_kernel(Batch b, SiftParams P) {
   extern __shared__ double dyn_lds[];
   __shared__ SiftShared<WG> sh;
-  persistent_body<WG>(b, P, sh, dyn_lds);
+  persistent_body<WG, AX>(b, P, sh, dyn_lds);
 }
 
 // Envelope pass without sifting (diagnostic; reference envelope_batch_kernel).
-template <int WG>
+template <int WG, class AX = IndexAxis>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void emd_envelope_kernel(EnvBatch e, SiftParams P) {
   extern __shared__ double dyn_lds[];
   __shared__ SiftShared<WG> sh;
+  using Pos = typename AX::Pos;
   const int tid = fresh_tid(), lane = tid & 63, w = uni(tid >> 6);
   constexpr int NW = WG / 64;
-  const WgMem m = workgroup_mem(dyn_lds, e.spill, e.spill_stride, P);
+  const WgMem<Pos> m = workgroup_mem<Pos>(dyn_lds, e.spill, e.spill_stride, P);
   const int bidx = blockIdx.x;
   const int N = e.N;
   const double* S = e.values + (int64_t)bidx * N;
+  SignalIO io{};
+  io.T = e.times ? e.times + (e.times_shared ? 0 : (int64_t)bidx * N) : nullptr;
+  const AX ax = make_axis(AX{}, io);
   for (int i = tid; i < N; i += WG) m.wk.imf[i] = S[i];
   __syncthreads();
-  DetectResult D = detect<WG>(m.wk, N, sh, m.tl, m.tg, P);
+  DetectResult D = detect<WG>(m.wk, N, sh, P);
   if (tid == 0) {
     e.n_extrema[2 * bidx] = D.nm;
     e.n_extrema[2 * bidx + 1] = D.nn;
@@ -1861,16 +1943,18 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void em
     if (tid == 0) e.degenerate[bidx] = 1;
     return;
   }
-  const Tile& t = D.in_lds ? m.tl : m.tg;
-  if (D.in_lds) build_envelopes<WG>(m.tl, sh, D, N, P, m.wk);
-  else build_envelopes<WG>(m.tg, sh, D, N, P, m.wk);
+  const TileT<Pos>& t = D.in_lds ? m.tl : m.tg;
+  if (D.in_lds) build_envelopes<WG>(m.tl, ax, sh, D, N, P, m.wk);
+  else build_envelopes<WG>(m.tg, ax, sh, D, N, P, m.wk);
   const int nb = P.nbsym;
   const int nblk = ceil_div(N, 64);
   for (int blk = w; blk < nblk; blk += NW) {
     const int i = blk * 64 + lane;
     if (i < N) {
-      e.max_env[(int64_t)bidx * N + i] = envelope_at(t, sh, 0, m.wk, blk, lane, i, nb);
-      e.min_env[(int64_t)bidx * N + i] = envelope_at(t, sh, 1, m.wk, blk, lane, i, 3 * nb + D.nm);
+      const Pos pi = ax.at(i);
+      e.max_env[(int64_t)bidx * N + i] = envelope_at(t, sh, 0, m.wk, blk, lane, pi, nb);
+      e.min_env[(int64_t)bidx * N + i] =
+          envelope_at(t, sh, 1, m.wk, blk, lane, pi, 3 * nb + D.nm);
     }
   }
   for (int side = 0; side < 2; ++side) {

@@ -13,12 +13,16 @@
 #define EMD_INST_FN EMD_PASTE(sift_instance_, EMD_WG)
 
 namespace emdk {
-// which: 0 plain, 1 persistent, 2 envelope
+// which: 0 plain, 1 persistent, 2 envelope (positions = sample indices); 3-5 the same
+// with positions from a time axis
 void* EMD_INST_FN(int which) {
   switch (which) {
-    case 0: return reinterpret_cast<void*>(&emd_plain_kernel<EMD_WG>);
-    case 1: return reinterpret_cast<void*>(&emd_persistent_kernel<EMD_WG>);
-    case 2: return reinterpret_cast<void*>(&emd_envelope_kernel<EMD_WG>);
+    case 0: return reinterpret_cast<void*>(&emd_plain_kernel<EMD_WG, IndexAxis>);
+    case 1: return reinterpret_cast<void*>(&emd_persistent_kernel<EMD_WG, IndexAxis>);
+    case 2: return reinterpret_cast<void*>(&emd_envelope_kernel<EMD_WG, IndexAxis>);
+    case 3: return reinterpret_cast<void*>(&emd_plain_kernel<EMD_WG, TimeAxis>);
+    case 4: return reinterpret_cast<void*>(&emd_persistent_kernel<EMD_WG, TimeAxis>);
+    case 5: return reinterpret_cast<void*>(&emd_envelope_kernel<EMD_WG, TimeAxis>);
   }
   return nullptr;
 }

@@ -62,6 +62,15 @@ PROTOTYPES = {
                                               _vp, _vp, _vp, _vp, _vp]),
     "emd_envelope_only": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
                                          _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "emd_run_padded_times": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                            _vp]),
+    "emd_run_ragged_times": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp,
+                                            ctypes.POINTER(EmdLaunchC), _vp, _vp, _vp, _vp, _vp,
+                                            _vp, _vp, _vp, _vp]),
+    "emd_envelope_only_times": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                               _vp, _vp, _vp]),
     "emd_max_resident": (ctypes.c_int, [_vp, ctypes.POINTER(EmdLaunchC), ctypes.c_int,
                                         _c_int_p]),
     "emd_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),

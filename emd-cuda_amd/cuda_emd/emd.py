@@ -60,10 +60,17 @@ class GPUBatchEMD:
         launcher = self._get_launcher()
         batch, N = signals.shape
         if T is not None:
-            Tn = T.detach().cpu().numpy() if hasattr(T, "detach") else np.asarray(T, np.float64)
-            Tn = np.asarray(Tn, dtype=np.float64)
-            d = np.diff(Tn)
-            T = (Tn - Tn[0]) / np.min(d) if Tn.size > 1 else Tn - Tn[0]
+            # the reference's normalisation (emd.py:83-88), per signal for a [B,N] axis
+            Tn = T.detach().cpu().numpy() if hasattr(T, "detach") else np.asarray(T)
+            if Tn.dtype != np.float64:
+                raise ValueError(f"T must be float64, got {Tn.dtype}")
+            if Tn.shape[-1:] != (N,) or Tn.ndim > 2:
+                raise ValueError(f"T must have shape ({N},) or ({batch}, {N}), got {Tn.shape}")
+            if N > 1:
+                d = np.diff(Tn, axis=-1)
+                T = (Tn - Tn[..., :1]) / np.min(d, axis=-1, keepdims=True)
+            else:
+                T = Tn - Tn[..., :1]
         imfs, n_imfs, overflow, debug_n, debug_extno, debug_nzm = launcher.run(signals, T)
         self.last_debug = (debug_n, debug_extno, debug_nzm)
         self.last_stats = launcher.last_stats

@@ -34,6 +34,14 @@ class EMDKernelConfig:
     tile of the one-workgroup-per-trace launch; 0 sizes it from ``max_extrema`` like the
     reference's shared-memory budget.  A sift whose knots outgrow the LDS tile keeps them
     in the workgroup's global tile instead, so the tile never changes results.
+
+    ``time_axis`` (addition) chooses what a caller's T means.  ``"index"`` (default):
+    PyEMD's EMD() in its 'simple' extrema mode, where the sample index is the time axis
+    whatever T is (a non-uniform T only warns).  ``"given"``: the reference kernel's use of
+    T (kernels.py:471-472, 528-552, 703): knots sit at T[idx], mirrored knots at
+    2 T[sym] - T[idx], and the envelopes are evaluated at T[i].  T is one axis [N] shared by
+    the batch or one per signal [B,N] (the per-signal extension kernels.py:826-828 names);
+    ``GPUBatchEMD`` normalises it as the reference does (emd.py:83-88).
     """
 
     max_extrema: int = 1024
@@ -42,6 +50,11 @@ class EMDKernelConfig:
     max_imfs: int = 16
     threads_per_block: int = 128
     tile_slots: int = 0
+    time_axis: str = "index"
+
+    def __post_init__(self):
+        if self.time_axis not in ("index", "given"):
+            raise ValueError(f"time_axis must be 'index' or 'given', got {self.time_axis!r}")
 
     @property
     def dtype(self) -> np.dtype:
@@ -178,30 +191,48 @@ class EMDBatchLauncher:
             self._h = None
 
     # -- helpers ----------------------------------------------------------------------
-    def _check_T(self, T, N):
+    def _check_T(self, T, B, N):
+        """Validate T.  Returns None (the sample index is the time axis) or, with
+        time_axis='given', (device tensor of the times, per_signal flag)."""
         if T is None:
-            return
-        Tn = T.detach().cpu().numpy() if hasattr(T, "detach") else np.asarray(T)
-        if Tn.dtype != np.float64:
+            return None
+        torch = _torch()
+        given = self.config.time_axis == "given"
+        Tt = T if isinstance(T, torch.Tensor) else None
+        Tn = np.asarray(T) if Tt is None else None
+        dt = Tt.dtype if Tt is not None else Tn.dtype
+        if dt not in (torch.float64, np.float64):
             raise ValueError(
                 f"This launcher was compiled for REAL_T=double (dtype float64), but got "
-                f"T.dtype={Tn.dtype}.")
-        if Tn.shape != (N,):
-            raise ValueError(f"T must have shape ({N},), got {Tn.shape}")
-        warn_nonuniform_T(Tn)
+                f"T.dtype={dt}.")
+        shape = tuple(Tt.shape) if Tt is not None else Tn.shape
+        if not given:
+            if shape != (N,):
+                raise ValueError(f"T must have shape ({N},), got {shape}")
+            warn_nonuniform_T(Tt.detach().cpu().numpy() if Tt is not None else Tn)
+            return None
+        if shape not in ((N,), (B, N)):
+            raise ValueError(f"T must have shape ({N},) or ({B}, {N}), got {shape}")
+        dev = Tt.detach().to("cuda").contiguous() if Tt is not None else \
+            torch.from_numpy(np.ascontiguousarray(Tn)).cuda()
+        if N > 1 and not bool(torch.all(dev[..., 1:] > dev[..., :-1])):
+            raise ValueError("T must increase strictly along each signal")
+        return dev, len(shape) == 2
 
     def run(self, signals, T=None) -> tuple:
         """Decompose ``signals`` [B,N] float64.  Returns (imfs [B,max_imfs,N], n_imfs [B],
         overflow [B], debug_n, debug_extno, debug_nzm [B,max_imfs]) as torch tensors; the
         per-trace statistics [B,4] (sifts, peak extrema, knot sum, flags) are kept in
-        ``last_stats``.  T is validated but, as in PyEMD's 'simple' extrema mode, the
-        sample index is the time axis."""
+        ``last_stats``.  With the default time_axis='index' T is validated but, as in PyEMD's
+        'simple' extrema mode, the sample index is the time axis; with time_axis='given' a
+        T ([N] or [B,N], normalised as GPUBatchEMD does) places the knots and the envelope
+        evaluation points, as in the reference kernel."""
         torch = _torch()
         s = _as_device_f64(signals, "signals")
         if s.dim() != 2:
             raise ValueError(f"signals must be [batch, N], got shape {tuple(s.shape)}")
         B, N = s.shape
-        self._check_T(T, N)
+        times = self._check_T(T, B, N)
         c = self.config
         dev = s.device
         imfs = torch.empty((B, c.max_imfs, N), dtype=torch.float64, device=dev)
@@ -210,20 +241,34 @@ class EMDBatchLauncher:
         n_imfs, overflow = torch.empty((B,), **i32), torch.empty((B,), **i32)
         dn, de, dz = (torch.empty((B, c.max_imfs), **i32) for _ in range(3))
         stats = torch.empty((B, 4), **i32)
-        _lib.check(_lib.load().emd_run_padded(
-            self._h, _ptr(s), B, N, _ptr(imfs), _ptr(n_imfs), _ptr(overflow), _ptr(dn),
-            _ptr(de), _ptr(dz), _ptr(stats), _stream()), "EMDBatchLauncher.run")
+        if times is None:
+            _lib.check(_lib.load().emd_run_padded(
+                self._h, _ptr(s), B, N, _ptr(imfs), _ptr(n_imfs), _ptr(overflow), _ptr(dn),
+                _ptr(de), _ptr(dz), _ptr(stats), _stream()), "EMDBatchLauncher.run")
+        else:
+            _lib.check(_lib.load().emd_run_padded_times(
+                self._h, _ptr(s), _ptr(times[0]), int(times[1]), B, N, _ptr(imfs),
+                _ptr(n_imfs), _ptr(overflow), _ptr(dn), _ptr(de), _ptr(dz), _ptr(stats),
+                _stream()), "EMDBatchLauncher.run")
         self.last_stats = stats
         return imfs, n_imfs, overflow, dn, de, dz
 
     def run_ragged(self, values, offsets, order=None, threads=None, tile_slots=0,
-                   wgs_per_cu=0, persistent=True, out_offsets=None) -> tuple:
+                   wgs_per_cu=0, persistent=True, out_offsets=None, times=None) -> tuple:
         """Decompose CSR traces (values, int64 offsets [B+1]).  ``order`` (int32 queue
         order) never changes outputs.  Returns (imfs flat, out_offsets [B] int64, n_imfs,
         overflow, debug_n, debug_extno, debug_nzm, stats); row k < n_imfs[s] of trace s is
-        imfs[out_offsets[s] + k*len(s) : ... + len(s)] (rows past n_imfs are scratch)."""
+        imfs[out_offsets[s] + k*len(s) : ... + len(s)] (rows past n_imfs are scratch).
+        ``times`` (float64, laid out like values: each trace's normalised, strictly
+        increasing time axis) runs the time-axis kernels whatever ``time_axis`` says."""
         torch = _torch()
         v = _as_device_f64(values, "values")
+        tv = None
+        if times is not None:
+            tv = _as_device_f64(times, "times")
+            if tv.numel() != v.numel():
+                raise ValueError(f"times must have {v.numel()} entries (like values), got "
+                                 f"{tv.numel()}")
         off = offsets if isinstance(offsets, torch.Tensor) else torch.from_numpy(
             np.ascontiguousarray(np.asarray(offsets, dtype=np.int64)))
         if off.dtype != torch.int64:
@@ -270,10 +315,16 @@ class EMDBatchLauncher:
                 raise ValueError(f"order must have {B} entries, got {order.numel()}")
         L = _lib.EmdLaunchC(threads or c.threads_per_block, tile_slots, wgs_per_cu,
                             1 if persistent else 0, B)
-        _lib.check(_lib.load().emd_run_ragged(
-            self._h, _ptr(v), _ptr(off), B, max_n, _ptr(order), ctypes.byref(L), _ptr(imfs),
-            _ptr(out_offsets), _ptr(n_imfs), _ptr(overflow), _ptr(dn), _ptr(de), _ptr(dz),
-            _ptr(stats), _stream()), "EMDBatchLauncher.run_ragged")
+        if tv is None:
+            _lib.check(_lib.load().emd_run_ragged(
+                self._h, _ptr(v), _ptr(off), B, max_n, _ptr(order), ctypes.byref(L),
+                _ptr(imfs), _ptr(out_offsets), _ptr(n_imfs), _ptr(overflow), _ptr(dn),
+                _ptr(de), _ptr(dz), _ptr(stats), _stream()), "EMDBatchLauncher.run_ragged")
+        else:
+            _lib.check(_lib.load().emd_run_ragged_times(
+                self._h, _ptr(v), _ptr(tv), _ptr(off), B, max_n, _ptr(order), ctypes.byref(L),
+                _ptr(imfs), _ptr(out_offsets), _ptr(n_imfs), _ptr(overflow), _ptr(dn),
+                _ptr(de), _ptr(dz), _ptr(stats), _stream()), "EMDBatchLauncher.run_ragged")
         self.last_stats = stats
         return imfs, out_offsets, n_imfs, overflow, dn, de, dz, stats
 
@@ -281,13 +332,14 @@ class EMDBatchLauncher:
         """Extrema + mirror + both envelope splines of the raw signals, no sifting
         (reference kernels.py:1374-1407).  Returns (max_env, min_env, n_extrema [B,2],
         degenerate [B], max_knot_pos, max_knot_val, min_knot_pos, min_knot_val [B,MBUF],
-        n_mirrored [B,2])."""
+        n_mirrored [B,2]).  T as in ``run`` (knot positions in time units with
+        time_axis='given')."""
         torch = _torch()
         s = _as_device_f64(signals, "signals")
         if s.dim() != 2:
             raise ValueError(f"signals must be [batch, N], got shape {tuple(s.shape)}")
         B, N = s.shape
-        self._check_T(T, N)
+        times = self._check_T(T, B, N)
         c = self.config
         mbuf = c.max_extrema + 2 * c.nbsym
         dev = s.device
@@ -296,10 +348,17 @@ class EMDBatchLauncher:
         n_extrema, degenerate = z(B, 2, dt=torch.int32), z(B, dt=torch.int32)
         mkp, mkv, nkp, nkv = z(B, mbuf), z(B, mbuf), z(B, mbuf), z(B, mbuf)
         n_mirrored = z(B, 2, dt=torch.int32)
-        _lib.check(_lib.load().emd_envelope_only(
-            self._h, _ptr(s), B, N, _ptr(max_env), _ptr(min_env), _ptr(n_extrema),
-            _ptr(degenerate), _ptr(mkp), _ptr(mkv), _ptr(nkp), _ptr(nkv), _ptr(n_mirrored),
-            _stream()), "EMDBatchLauncher.run_envelope_only")
+        if times is None:
+            _lib.check(_lib.load().emd_envelope_only(
+                self._h, _ptr(s), B, N, _ptr(max_env), _ptr(min_env), _ptr(n_extrema),
+                _ptr(degenerate), _ptr(mkp), _ptr(mkv), _ptr(nkp), _ptr(nkv),
+                _ptr(n_mirrored), _stream()), "EMDBatchLauncher.run_envelope_only")
+        else:
+            _lib.check(_lib.load().emd_envelope_only_times(
+                self._h, _ptr(s), _ptr(times[0]), int(times[1]), B, N, _ptr(max_env),
+                _ptr(min_env), _ptr(n_extrema), _ptr(degenerate), _ptr(mkp), _ptr(mkv),
+                _ptr(nkp), _ptr(nkv), _ptr(n_mirrored), _stream()),
+                "EMDBatchLauncher.run_envelope_only")
         return max_env, min_env, n_extrema, degenerate, mkp, mkv, nkp, nkv, n_mirrored
 
     def compact(self, imfs, n_imfs, offsets=None, N: int | None = None, out_offsets=None):

@@ -95,6 +95,29 @@ int emd_run_ragged(emd_handle* h, const double* values, const int64_t* offsets, 
                    const int64_t* out_offsets, int* n_imfs, int* overflow, int* debug_n,
                    int* debug_extno, int* debug_nzm, int* stats, void* stream);
 
+/* emd_run_padded with the reference's time axis: replaces EMDBatchLauncher.run(signals, T)
+ * -> emd_batch_kernel for a T that is not arange(N) (kernels.py:1343-1372; T used at
+ * kernels.py:471-472, 528-552, 703).  times holds the normalised axis, (T - T[0]) /
+ * min(diff T) as GPUBatchEMD computes it (emd.py:83-88), strictly increasing: one shared
+ * row [N] (times_per_signal = 0, the reference's layout) or one row per signal [B,N]
+ * (times_per_signal = 1, the per-signal extension kernels.py:826-828 names).  Knots sit at
+ * times[idx], mirrored knots at 2*times[sym] - times[idx], and the envelopes are evaluated
+ * at times[i]; extrema are still found by sample index.  Same outputs as emd_run_padded. */
+int emd_run_padded_times(emd_handle* h, const double* signals, const double* times,
+                         int times_per_signal, int B, int N, double* imfs, int* n_imfs,
+                         int* overflow, int* debug_n, int* debug_extno, int* debug_nzm,
+                         int* stats, void* stream);
+
+/* emd_run_ragged with a time axis per trace: times is laid out like values (trace s's
+ * normalised times at times[offsets[s] .. offsets[s+1])).  No reference counterpart (the
+ * reference takes one shared N); its per-trace semantics are those of
+ * emd_run_padded_times. */
+int emd_run_ragged_times(emd_handle* h, const double* values, const double* times,
+                         const int64_t* offsets, int B, int max_n, const int* order,
+                         const emd_launch* launch, double* imfs, const int64_t* out_offsets,
+                         int* n_imfs, int* overflow, int* debug_n, int* debug_extno,
+                         int* debug_nzm, int* stats, void* stream);
+
 /* emd_run_ragged without any device allocation.  No counterpart in the reference: for
  * callers that cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  Same arguments, outputs and queue semantics
@@ -122,6 +145,15 @@ int emd_envelope_only(emd_handle* h, const double* signals, int B, int N, double
                       double* min_env, int* n_extrema, int* degenerate, double* max_kpos,
                       double* max_kval, double* min_kpos, double* min_kval, int* n_mirrored,
                       void* stream);
+
+/* emd_envelope_only with the reference's time axis (run_envelope_only(signals, T),
+ * kernels.py:1374-1407 with T used as in emd_run_padded_times): times [N] shared
+ * (times_per_signal = 0) or [B,N].  Knot positions are reported in time units. */
+int emd_envelope_only_times(emd_handle* h, const double* signals, const double* times,
+                            int times_per_signal, int B, int N, double* max_env,
+                            double* min_env, int* n_extrema, int* degenerate, double* max_kpos,
+                            double* max_kval, double* min_kpos, double* min_kval,
+                            int* n_mirrored, void* stream);
 
 /* Largest number of workgroups of a launch shape resident per CU (occupancy query for
  * the autotuner).  No reference counterpart (CUDA occupancy reasoning lives in

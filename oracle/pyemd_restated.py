@@ -190,17 +190,35 @@ def find_extrema_reference(S: np.ndarray):
 # ---------------------------------------------------------------------------------------
 # Boundary mirroring
 # ---------------------------------------------------------------------------------------
-def prepare_points_simple(S: np.ndarray, ind_max: np.ndarray, ind_min: np.ndarray, nbsym: int = 2):
-    """PyEMD ``EMD._prepare_points_simple`` with T = arange(N): mirror ``nbsym`` extrema
-    about the chosen symmetry points, re-mirror about the signal edge when the mirrored
-    points do not reach it, then drop adjacent knots with equal positions.
+def normalize_time(T) -> np.ndarray:
+    """The reference's time normalisation (emd.py:83-88, PyEMD ``_normalize_time``):
+    ``(T - T[0]) / min(diff T)``.  T must increase strictly (the reference divides by the
+    smallest step, and its mirror and interval search assume an increasing axis)."""
+    T = np.asarray(T, dtype=np.float64)
+    if T.ndim != 1:
+        raise ValueError("T must be one-dimensional")
+    if T.size > 1:
+        d = np.diff(T)
+        if not np.all(d > 0):
+            raise ValueError("T must increase strictly")
+        return (T - T[0]) / np.min(d)
+    return T - T[0] if T.size else T
 
-    Reference counterpart: mirror_one_side (kernels.py:360-558).  Returns
-    (max_knots [2,k], min_knots [2,k]) rows (position, value).  The PyEMD
-    "Left/Right edge BUG" exceptions are unreachable for T = arange(N); they are kept.
+
+def prepare_points_simple(S: np.ndarray, ind_max: np.ndarray, ind_min: np.ndarray, nbsym: int = 2,
+                          T: np.ndarray | None = None):
+    """PyEMD ``EMD._prepare_points_simple``: mirror ``nbsym`` extrema about the chosen
+    symmetry points, re-mirror about the signal edge when the mirrored points do not reach
+    it, then drop adjacent knots with equal positions.  T: the (normalised) time axis;
+    None = arange(N), PyEMD's simple mode.
+
+    Reference counterpart: mirror_one_side (kernels.py:360-558), which places knots at
+    T[idx] and mirrored knots at 2 T[sym] - T[idx] (kernels.py:471-472, 528-552).  Returns
+    (max_knots [2,k], min_knots [2,k]) rows (position, value).  The PyEMD "Left/Right
+    edge BUG" exceptions are unreachable for an increasing T with T[0] = 0; they are kept.
     """
     N = S.size
-    T = np.arange(N, dtype=np.float64)
+    T = np.arange(N, dtype=np.float64) if T is None else np.asarray(T, dtype=np.float64)
     ind_min = np.asarray(ind_min, dtype=int)
     ind_max = np.asarray(ind_max, dtype=int)
     end_min, end_max = len(ind_min), len(ind_max)
@@ -445,10 +463,14 @@ class EMDResult:
 
 def emd(S, max_imf: int = -1, max_iteration: int = 1000, nbsym: int = 2,
         max_extrema: int | None = None, debug_slots: int = 16,
-        mode: str = "pyemd") -> EMDResult:
+        mode: str = "pyemd", T=None) -> EMDResult:
     """PyEMD ``EMD.emd`` with the default stopping rule (FIXE = FIXE_H = 0).
 
-    T is always arange(N) (PyEMD replaces T in 'simple' mode).  Reference counterpart:
+    T = None: the time axis is arange(N), as PyEMD's 'simple' mode always makes it.  A
+    given T is normalised (``normalize_time``) and used the way the reference kernel uses
+    its time axis: knot positions, mirrored positions and the envelope evaluation points
+    (emd.py:83-88; kernels.py:471-472, 528-552, 703); extrema are still found by sample
+    index.  Reference counterpart:
     emd_batch_kernel (kernels.py:834-1080); debug records follow kernels.py:901-928,
     972-979 (sift count at convergence, = max_iteration with extno/nzm -1 when capped,
     -1 for slots never reached).  ``mode='reference'`` applies the reference kernel's own
@@ -460,7 +482,12 @@ def emd(S, max_imf: int = -1, max_iteration: int = 1000, nbsym: int = 2,
     find = find_extrema_reference if ref else find_extrema_simple
     S = np.asarray(S, dtype=np.float64)
     N = S.size
-    T = np.arange(N, dtype=np.float64)
+    if T is None:
+        T = np.arange(N, dtype=np.float64)
+    else:
+        if np.asarray(T).shape != (N,):
+            raise ValueError(f"T has shape {np.asarray(T).shape}, the signal {S.shape}")
+        T = normalize_time(T)
     dbg_n = np.full(debug_slots, -1, np.int64)
     dbg_e = np.full(debug_slots, -1, np.int64)
     dbg_z = np.full(debug_slots, -1, np.int64)
@@ -514,7 +541,7 @@ def emd(S, max_imf: int = -1, max_iteration: int = 1000, nbsym: int = 2,
             imax, imin, nzm = extrema(imf, None if (imfNo == 0 and n == 1) else (imfNo, n))
             extNo = len(imax) + len(imin)
             if extNo > 2 and len(imax) >= 1 and len(imin) >= 1:
-                max_k, min_k = prepare_points_simple(imf, imax, imin, nbsym)
+                max_k, min_k = prepare_points_simple(imf, imax, imin, nbsym, T)
                 max_env = spline_points(T, max_k, mode)
                 min_env = spline_points(T, min_k, mode)
                 total_sifts += 1
@@ -559,15 +586,16 @@ def emd(S, max_imf: int = -1, max_iteration: int = 1000, nbsym: int = 2,
                      knot_sum, cap_hit, ties)
 
 
-def envelopes(S, nbsym: int = 2, max_extrema: int | None = None):
+def envelopes(S, nbsym: int = 2, max_extrema: int | None = None, T=None):
     """One envelope pass without sifting (the reference's diagnostic
     ``envelope_batch_kernel``, kernels.py:1100-1166): extrema, mirror, both splines.
+    T as in ``emd`` (None = arange(N); a given T is normalised).
 
     Returns dict(max_env, min_env, n_max, n_min, degenerate, max_knots, min_knots).
     """
     S = np.asarray(S, dtype=np.float64)
     N = S.size
-    T = np.arange(N, dtype=np.float64)
+    T = np.arange(N, dtype=np.float64) if T is None else normalize_time(T)
     imax, imin, _ = find_extrema_simple(S)
     if max_extrema is not None:
         imax, imin = imax[:max_extrema], imin[:max_extrema]
@@ -577,7 +605,7 @@ def envelopes(S, nbsym: int = 2, max_extrema: int | None = None):
     if len(imax) + len(imin) <= 2 or len(imax) < 1 or len(imin) < 1:
         out["degenerate"] = 1
         return out
-    mk, nk = prepare_points_simple(S, imax, imin, nbsym)
+    mk, nk = prepare_points_simple(S, imax, imin, nbsym, T)
     out["max_env"] = spline_points(T, mk)
     out["min_env"] = spline_points(T, nk)
     out["max_knots"], out["min_knots"] = mk, nk
@@ -585,13 +613,18 @@ def envelopes(S, nbsym: int = 2, max_extrema: int | None = None):
 
 
 def emd_batch(signals, max_imfs: int = 16, max_iteration: int = 1000, nbsym: int = 2,
-              max_extrema: int | None = None):
+              max_extrema: int | None = None, T=None):
     """Batch driver returning the reference's padded outputs
     (imfs [B,max_imfs,N], n_imfs, overflow, debug_n/extno/nzm [B,max_imfs]) plus the
     per-signal EMDResult list.  The IMF cap is the reference's ``max_imfs - 1``
-    (kernels.py:894) expressed as PyEMD ``max_imf``."""
+    (kernels.py:894) expressed as PyEMD ``max_imf``.  T: None, one time axis [N] shared
+    by the batch, or one per signal [B,N] (each normalised on its own)."""
     signals = np.atleast_2d(np.asarray(signals, dtype=np.float64))
     B, N = signals.shape
+    if T is not None:
+        T = np.asarray(T, dtype=np.float64)
+        if T.shape not in ((N,), (B, N)):
+            raise ValueError(f"T has shape {T.shape}; expected ({N},) or ({B}, {N})")
     imfs = np.zeros((B, max_imfs, N))
     n_imfs = np.zeros(B, np.int32)
     ovf = np.zeros(B, np.int32)
@@ -600,8 +633,9 @@ def emd_batch(signals, max_imfs: int = 16, max_iteration: int = 1000, nbsym: int
     dz = np.full((B, max_imfs), -1, np.int32)
     results = []
     for b in range(B):
+        Tb = None if T is None else (T if T.ndim == 1 else T[b])
         r = emd(signals[b], max_imf=max_imfs - 1, max_iteration=max_iteration, nbsym=nbsym,
-                max_extrema=max_extrema, debug_slots=max_imfs)
+                max_extrema=max_extrema, debug_slots=max_imfs, T=Tb)
         k = r.n_imfs
         imfs[b, :k] = r.imfs
         n_imfs[b] = k

@@ -13,8 +13,10 @@
 //
 // usage: sift_emu <input file> <output file>
 // input (little-endian): int32 magic 0x454d5531, B, wg, tile_slots, resident, persistent,
-//   max_extrema, nbsym, max_iteration, max_imfs, has_order; int64 n_values;
-//   int64 offsets[B+1]; double values[n_values]; int32 order[B] if has_order
+//   max_extrema, nbsym, max_iteration, max_imfs, has_order, has_times; int64 n_values;
+//   int64 offsets[B+1]; double values[n_values]; int32 order[B] if has_order;
+//   double times[n_values] if has_times (each trace's normalised time axis, laid out like
+//   values: the time-axis kernels run)
 // output: int32 n_imfs[B], overflow[B], debug_n/extno/nzm [B*max_imfs], stats[B*4];
 //   double imfs[n_values*max_imfs] (row k of trace s at offsets[s]*max_imfs + k*len(s))
 // exit status: 0 done, 2 bad input, 3 a barrier not reached by every lane (printed),
@@ -25,13 +27,13 @@
 
 namespace {
 
-int64_t region_bytes(int slots) {
-  return ((int64_t)slots * emdk::kSlotBytes + 255) / 256 * 256;
+int64_t region_bytes(int slots, int slot_bytes) {
+  return ((int64_t)slots * slot_bytes + 255) / 256 * 256;
 }
 
-size_t dyn_lds(int tile_slots, int max_n) {
+size_t dyn_lds(int tile_slots, int max_n, int slot_bytes) {
   const size_t nbcap = (size_t)(max_n + 63) / 64;
-  return nbcap * emdk::kWorkBlockBytes + (size_t)tile_slots * emdk::kSlotBytes;
+  return nbcap * emdk::kWorkBlockBytes + (size_t)tile_slots * slot_bytes;
 }
 
 struct Input {
@@ -39,15 +41,16 @@ struct Input {
   std::vector<int64_t> offsets;
   std::vector<double> values;
   std::vector<int> order;
+  std::vector<double> times;
 };
 
-template <int WG>
+template <int WG, class AX>
 void run_workgroup(int bid, int grid, const emdk::Batch& b, const emdk::SiftParams& P,
                    bool persistent) {
   emu::Group g(WG);
   // LDS starts as a NaN pattern each launch: a read of an element no lane wrote shows up
   // as a NaN in the outputs
-  std::vector<double> lds(dyn_lds(P.tile_slots, P.max_n) / 8 + 1,
+  std::vector<double> lds(dyn_lds(P.tile_slots, P.max_n, AX::kSlot) / 8 + 1,
                           std::numeric_limits<double>::quiet_NaN());
   auto* sh = new emdk::SiftShared<WG>;
   std::memset((void*)sh, 0xA5, sizeof(*sh));
@@ -62,14 +65,14 @@ void run_workgroup(int bid, int grid, const emdk::Batch& b, const emdk::SiftPara
   }
   emu::run_group(g, [&](emu::Lane&) {
     if (persistent)
-      emdk::persistent_body<WG>(b, P, *sh, lds.data());
+      emdk::persistent_body<WG, AX>(b, P, *sh, lds.data());
     else
-      emdk::plain_body<WG>(b, P, *sh, lds.data());
+      emdk::plain_body<WG, AX>(b, P, *sh, lds.data());
   });
   delete sh;
 }
 
-template <int WG>
+template <int WG, class AX>
 void run(const Input& in, const emdk::Batch& b0, const emdk::SiftParams& P, int64_t stride) {
   emdk::Batch b = b0;
   if (in.persistent) {
@@ -84,7 +87,7 @@ void run(const Input& in, const emdk::Batch& b0, const emdk::SiftParams& P, int6
     b.base = 0;
     std::vector<std::thread> wgs;
     for (int w = 0; w < grid; ++w)
-      wgs.emplace_back([&, w] { run_workgroup<WG>(w, grid, b, P, true); });
+      wgs.emplace_back([&, w] { run_workgroup<WG, AX>(w, grid, b, P, true); });
     for (auto& th : wgs) th.join();
     return;
   }
@@ -100,7 +103,7 @@ void run(const Input& in, const emdk::Batch& b0, const emdk::SiftParams& P, int6
   for (int first = 0; first < in.B; first += conc) {
     std::vector<std::thread> wgs;
     for (int w = first; w < std::min(in.B, first + conc); ++w)
-      wgs.emplace_back([&, w] { run_workgroup<WG>(w, in.B, b, P, false); });
+      wgs.emplace_back([&, w] { run_workgroup<WG, AX>(w, in.B, b, P, false); });
     for (auto& th : wgs) th.join();
   }
 }
@@ -119,10 +122,10 @@ int main(int argc, char** argv) {
   }
   FILE* f = std::fopen(argv[1], "rb");
   if (!f) return 2;
-  int32_t hdr[11];
+  int32_t hdr[12];
   int64_t n_values = 0;
   Input in;
-  if (!rd(f, hdr, 11) || hdr[0] != 0x454d5531 || !rd(f, &n_values, 1)) return 2;
+  if (!rd(f, hdr, 12) || hdr[0] != 0x454d5531 || !rd(f, &n_values, 1)) return 2;
   in.B = hdr[1];
   in.wg = hdr[2];
   in.tile = hdr[3];
@@ -139,6 +142,11 @@ int main(int argc, char** argv) {
     in.order.resize(in.B);
     if (!rd(f, in.order.data(), in.B)) return 2;
   }
+  const bool timed = hdr[11] != 0;
+  if (timed) {
+    in.times.resize(n_values);
+    if (!rd(f, in.times.data(), n_values)) return 2;
+  }
   std::fclose(f);
   if (in.B < 0 || in.offsets[0] < 0 || in.offsets[in.B] > n_values) return 2;
   for (int s = 0; s < in.B; ++s)
@@ -154,11 +162,12 @@ int main(int argc, char** argv) {
   const int cap = 2 * std::min(in.max_extrema, std::max(max_n, 1)) + 4 * in.nbsym;
   int tile = in.tile > 0 ? std::min(in.tile, cap) : cap;
   const long lds_max = 163840;
+  const int slot_bytes = timed ? emdk::kSlotBytesTime : emdk::kSlotBytes;
   const long fit = (lds_max - (long)(in.wg == 64    ? sizeof(emdk::SiftShared<64>)
                                      : in.wg == 128 ? sizeof(emdk::SiftShared<128>)
                                                     : sizeof(emdk::SiftShared<256>)) -
-                    (long)dyn_lds(0, max_n)) /
-                   emdk::kSlotBytes;
+                    (long)dyn_lds(0, max_n, slot_bytes)) /
+                   slot_bytes;
   tile = std::max(0, std::min<int>(tile, (int)std::max(0L, fit)));
   emdk::SiftParams P;
   P.max_extrema = in.max_extrema;
@@ -169,7 +178,7 @@ int main(int argc, char** argv) {
   P.spill_slots = tile < cap ? cap : 0;
   P.max_n = std::max(max_n, 1);
   P.zero_fill = 0;
-  const int64_t stride = P.spill_slots ? region_bytes(P.spill_slots) : 0;
+  const int64_t stride = P.spill_slots ? region_bytes(P.spill_slots, slot_bytes) : 0;
 
   const int K = in.max_imfs;
   std::vector<double> imfs((size_t)std::max<int64_t>(n_values * K, 1),
@@ -192,13 +201,18 @@ int main(int argc, char** argv) {
   b.dbg_e = de.data();
   b.dbg_z = dz.data();
   b.stats = stats.data();
+  b.times = timed ? in.times.data() : nullptr;
+  b.times_shared = 0;
   std::fprintf(stderr, "sift_emu: B=%d wg=%d tile=%d (cap %d) spill_slots=%d "
                "resident=%d persistent=%d max_n=%d\n", in.B, in.wg, tile, cap, P.spill_slots,
                in.resident, in.persistent, max_n);
-  switch (in.wg) {
-    case 64: run<64>(in, b, P, stride); break;
-    case 128: run<128>(in, b, P, stride); break;
-    case 256: run<256>(in, b, P, stride); break;
+  switch (in.wg * 2 + (timed ? 1 : 0)) {
+    case 128: run<64, emdk::IndexAxis>(in, b, P, stride); break;
+    case 256: run<128, emdk::IndexAxis>(in, b, P, stride); break;
+    case 512: run<256, emdk::IndexAxis>(in, b, P, stride); break;
+    case 129: run<64, emdk::TimeAxis>(in, b, P, stride); break;
+    case 257: run<128, emdk::TimeAxis>(in, b, P, stride); break;
+    case 513: run<256, emdk::TimeAxis>(in, b, P, stride); break;
     default: std::fprintf(stderr, "workgroup size %d not emulated\n", in.wg); return 2;
   }
   FILE* o = std::fopen(argv[2], "wb");

@@ -38,8 +38,9 @@ def emu():
 
 
 def run_emu(exe, traces, wg, tile=0, resident=2, persistent=True, order=None,
-            max_extrema=1024, max_imfs=16, max_iteration=1000, timeout=1800):
-    """Decompose `traces` (list of 1-D float64) in the emulator; returns the parity dict."""
+            max_extrema=1024, max_imfs=16, max_iteration=1000, timeout=1800, times=None):
+    """Decompose `traces` (list of 1-D float64) in the emulator; returns the parity dict.
+    times: one normalised time axis per trace (the time-axis kernels), or None."""
     lengths = np.array([len(t) for t in traces], np.int64)
     offsets = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
     values = np.concatenate(traces).astype(np.float64) if len(traces) else np.zeros(0)
@@ -48,12 +49,15 @@ def run_emu(exe, traces, wg, tile=0, resident=2, persistent=True, order=None,
         fi, fo = os.path.join(d, "in"), os.path.join(d, "out")
         with open(fi, "wb") as f:
             np.array([MAGIC, B, wg, tile, resident, int(persistent), max_extrema, 2,
-                      max_iteration, K, order is not None], np.int32).tofile(f)
+                      max_iteration, K, order is not None, times is not None],
+                     np.int32).tofile(f)
             np.array([values.size], np.int64).tofile(f)
             offsets.tofile(f)
             values.tofile(f)
             if order is not None:
                 np.asarray(order, np.int32).tofile(f)
+            if times is not None:
+                np.concatenate([np.asarray(t, np.float64) for t in times]).tofile(f)
         env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:handle_abort=1",
                    UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
         r = subprocess.run([exe, fi, fo], capture_output=True, text=True, timeout=timeout,
@@ -174,3 +178,23 @@ def test_long_spline_systems(emu):
     _assert_parity(traces, out, "emu:long-systems")
     ref = run_emu(emu, traces, 64, resident=2, persistent=False)
     _same(out, ref)
+
+
+def test_time_axis(emu):
+    """The time-axis kernels (the reference's T: knots at T[idx], mirrored knots at
+    2 T[sym] - T[idx], envelopes evaluated at T[i]) on jittered sampling, against the
+    oracle given the same T: persistent at 64 lanes, and plain at 128 lanes with a 40-slot
+    LDS tile so the knots go to the global tile of 32-byte slots."""
+    from cuda_emd import synth
+    rng = np.random.default_rng(11)
+    traces = [np.ascontiguousarray(x) for x in synth.chirp_mix(3, 300, seed=13)]
+    T = [np.cumsum(rng.uniform(0.6, 1.4, len(x))) for x in traces]
+    ora = [O.emd(x, max_imf=15, max_extrema=1024, T=t) for x, t in zip(traces, T)]
+    times = [O.normalize_time(t) for t in T]
+    out = run_emu(emu, traces, 64, resident=2, persistent=True, times=times)
+    bad, _ = compare(list(traces), out, ora, 16, label="emu:time-axis")
+    assert not bad
+    small = run_emu(emu, traces, 128, tile=40, resident=3, persistent=False, times=times)
+    bad, _ = compare(list(traces), small, ora, 16, label="emu:time-axis-global-tile")
+    assert not bad
+    _same(out, small)
