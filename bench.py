@@ -65,32 +65,37 @@ def measured_traffic(name="emd_sift_kernels_seismic_16384x2001.md"):
     return profile_values(name).get("traffic_bytes_per_launch")
 
 
-def _cpu_one(x):
+def _cpu_one(x, T=None):
     from oracle import pyemd_restated as O
-    return O.emd(x, max_imf=15).n_imfs
+    return O.emd(x, max_imf=15, T=T).n_imfs
 
 
-def _cpu_record(x):
+def _cpu_record(x, T=None):
     from oracle import pyemd_restated as O
-    return O.emd(x, max_imf=15)
+    return O.emd(x, max_imf=15, T=T)
 
 
-def cpu_baseline(x: np.ndarray, n: int):
+def cpu_baseline(x: np.ndarray, n: int, T=None):
     """The oracle timed on the first n traces with a process pool (the reported CPU
     baseline), then run again untimed to return its full records for the parity check of
-    the GPU output on the same traces."""
+    the GPU output on the same traces.  T: a time axis shared by the traces (the
+    reference's T), or None."""
+    import functools
     import multiprocessing as mp
     cores = min(16, os.cpu_count() or 1)
     sample = x[:n]
+    one = functools.partial(_cpu_one, T=T)
+    rec = functools.partial(_cpu_record, T=T)
     ctx = mp.get_context("fork")
     with ctx.Pool(cores) as pool:
-        pool.map(_cpu_one, sample[:cores])  # warm the workers (imports)
+        pool.map(one, sample[:cores])  # warm the workers (imports)
         t0 = time.perf_counter()
-        pool.map(_cpu_one, sample, chunksize=max(1, n // (4 * cores)))
+        pool.map(one, sample, chunksize=max(1, n // (4 * cores)))
         dt = time.perf_counter() - t0
-        records = pool.map(_cpu_record, sample, chunksize=max(1, n // (4 * cores)))
+        records = pool.map(rec, sample, chunksize=max(1, n // (4 * cores)))
+    axis = "" if T is None else " on a shared jittered time axis"
     return {"value": n / dt, "unit": "traces/s", "cores": cores, "kind": "port",
-            "sample": f"first {n} of the {len(x)} seismic traces (N={x.shape[1]}), "
+            "sample": f"first {n} of the {len(x)} seismic traces (N={x.shape[1]}){axis}, "
                       f"NumPy/SciPy PyEMD EMD() restatement, multiprocessing pool of {cores}"}, records
 
 
@@ -454,6 +459,85 @@ def bench_segy(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_timeaxis(args):
+    """The time-axis kernels (EMDKernelConfig(time_axis='given'), the reference's use of
+    T): the seismic traces (16384 x N=2001, seed 1) on one jittered axis shared by the
+    batch (steps of 4 ms x U(0.8, 1.2), seed 7), decomposed by the padded time-axis
+    entry point (the reference's launch model, one workgroup per trace) at the packed
+    path's shape (256 lanes, the largest knot tile that keeps 4 workgroups per CU), timed
+    with HIP events beside the index kernel at its own packed shape on the same traces."""
+    import torch
+    from cuda_emd import EMDKernelConfig, synth
+    x = synth.seismic_traces(args.batch, args.n, seed=1)
+    B, N = x.shape
+    rng = np.random.default_rng(7)
+    T = 0.004 * np.cumsum(rng.uniform(0.8, 1.2, N))
+    Tn = (T - T[0]) / np.min(np.diff(T))  # the reference's normalisation (emd.py:83-88)
+    cpu, records = (None, None) if args.no_cpu else cpu_baseline(x, args.cpu_sample, T=T)
+    dx = torch.from_numpy(x).cuda()
+    dT = torch.from_numpy(Tn).cuda()
+    lengths = np.full(B, N, dtype=np.int64)
+
+    def timed(cfg, with_axis):
+        L = cfg.build()
+        L.set_timing(True)
+        out = None
+        for _ in range(args.warmup):
+            out = L.run(dx, dT if with_axis else None)
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(args.steps):
+            out = L.run(dx, dT if with_axis else None)
+            torch.cuda.synchronize()
+            ms.append(L.kernel_time_ms())
+        return out, L.last_stats, float(np.mean(ms))
+
+    lds = 40960 - EMDKernelConfig(threads_per_block=256).static_shared_mem_bytes() \
+        - 568 * ((N + 63) // 64)
+    tile_t, tile_i = lds // 32, lds // 28
+    out_t, st_t, ms_t = timed(EMDKernelConfig(time_axis="given", threads_per_block=256,
+                                              tile_slots=tile_t), True)
+    _, _, ms_i = timed(EMDKernelConfig(threads_per_block=256, tile_slots=tile_i), False)
+    n_imfs = out_t[1].cpu().numpy()
+    stats = st_t.cpu().numpy()
+    flops = algorithmic_flops(stats, n_imfs, lengths)
+    achieved = flops / (ms_t * 1e-3) / 1e12
+    line = {
+        "metric": "traces/sec fp64 N=2001 on a shared non-uniform time axis (the reference's "
+                  "T), one workgroup per trace, 1 GPU",
+        "value": B / (ms_t * 1e-3), "unit": "traces/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_t, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"seismic traces, {B} x N={N}, seed 1, jittered time axis "
+                               "(4 ms x U(0.8, 1.2), seed 7)", "traces": B, "N": N,
+                   "threads": 256, "tile_slots": tile_t},
+        "roofline": {"bound": "valu", "roof": "fp64 VALU (78.6 TFLOP/s)",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_launch": flops, "kernel_ms": ms_t},
+        "index_axis_same_shape": {"kernel_ms": ms_i, "traces_per_s": B / (ms_i * 1e-3),
+                                  "tile_slots": tile_i},
+        "sifts_per_trace": float(stats[:, 0].mean()),
+    }
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from parity import compare
+        n = len(records)
+        gpu = dict(imfs=out_t[0].cpu().numpy()[:n], n_imfs=n_imfs[:n],
+                   overflow=out_t[2].cpu().numpy()[:n], debug_n=out_t[3].cpu().numpy()[:n],
+                   debug_extno=out_t[4].cpu().numpy()[:n],
+                   debug_nzm=out_t[5].cpu().numpy()[:n], stats=stats[:n])
+        bad, worst = compare(list(x[:n]), gpu, records, label="bench:timeaxis")
+        traces_bad = sorted({m[0] for m in bad})
+        unexplained = sorted({m[0] for m in bad if not m[2]})
+        line["parity_vs_oracle"] = {"traces": n, "mismatched_traces": len(traces_bad),
+                                    "tie_explained": len(traces_bad) - len(unexplained),
+                                    "unexplained": len(unexplained),
+                                    "worst_imf_err_rel": worst}
+    print(json.dumps(line), flush=True)
+
+
 def hilbert_work(lengths: np.ndarray, n_imfs: np.ndarray, n_fft: int):
     """(algorithmic bytes, algorithmic flops) of one Hilbert launch.  Bytes: each IMF sample
     is read once and its amplitude, phase and frequency written once (32 B).  Flops: per
@@ -625,7 +709,7 @@ def main():
                     help="seismic traces (seed 3) the cost model is trained on before timing")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--workload", choices=["seismic", "ragged", "costmodel", "stress", "segy",
-                                           "hilbert"],
+                                           "hilbert", "timeaxis"],
                     default="seismic",
                     help="seismic: the headline 16384 x N=2001 traces; ragged: the ragged mix "
                          "(N log-uniform in [256, 8192], seed 2; --batch sets B, default 65536); "
@@ -634,7 +718,9 @@ def main():
                          "stress: 262144 x N=4096 (seed 5) streamed from host in captured "
                          "chunks, max_extrema 2048, compacted output; segy: decode of a "
                          "16384 x N=2001 IBM-float SEG-Y image in HBM; hilbert: amplitude, "
-                         "phase and frequency of the seismic traces' compacted IMFs")
+                         "phase and frequency of the seismic traces' compacted IMFs; "
+                         "timeaxis: the seismic traces on a shared jittered time axis "
+                         "through the time-axis kernels (the reference's T)")
     ap.add_argument("--train-total", type=int, default=200000)
     ap.add_argument("--eval-total", type=int, default=262144)
     ap.add_argument("--tune-cache", default=None, help="autotuner JSON cache (ragged)")
@@ -664,6 +750,8 @@ def main():
         return bench_segy(args)
     if args.workload == "hilbert":
         return bench_hilbert(args)
+    if args.workload == "timeaxis":
+        return bench_timeaxis(args)
 
     from cuda_emd import synth
 
