@@ -139,3 +139,21 @@ def test_envelope_only_with_axis():
         assert np.array_equal(mkp[b, :k[0]], r["max_knots"][0])
         assert np.array_equal(nkp[b, :k[1]], r["min_knots"][0])
         assert np.array_equal(mkv[b, :k[0]], r["max_knots"][1])
+
+
+def test_edge_signals_and_short_lengths_on_an_axis():
+    """The index-axis edge signals (plateaus, exact zeros, steps, spikes, square waves)
+    and lengths 0-130 on jittered axes: the plateau rules, the small spline kinds and the
+    degenerate traces with time positions, against the oracle."""
+    from test_gpu_parity import _edge_signals
+    rng = np.random.default_rng(25)
+    sigs = list(_edge_signals().values())
+    T = _jittered(rng, sigs[0].size)
+    ora = [O.emd(s, max_imf=15, T=T) for s in sigs]
+    x = np.stack(sigs)
+    _assert_parity(x, _gpu_emd(x, T), ora, "time-axis edge signals")
+    for N in (0, 1, 2, 3, 4, 5, 7, 64, 65, 130):
+        x = rng.standard_normal((3, N))
+        T = np.stack([_jittered(rng, N) for _ in range(3)]) if N else np.zeros((3, 0))
+        ora = [O.emd(s, max_imf=15, T=t) for s, t in zip(x, T)]
+        _assert_parity(x, _gpu_emd(x, T), ora, f"time-axis N={N}")

@@ -198,3 +198,21 @@ def test_time_axis(emu):
     bad, _ = compare(list(traces), small, ora, 16, label="emu:time-axis-global-tile")
     assert not bad
     _same(out, small)
+
+
+def test_time_axis_edge_signals(emu):
+    """Plateaus, exact zeros and short traces on jittered axes in the emulation (the
+    serial plateau rules, the small spline kinds and degenerate traces with time
+    positions), against the oracle."""
+    rng = np.random.default_rng(12)
+    n = np.arange(257, dtype=np.float64)
+    traces = [np.floor(np.sin(2 * np.pi * n / 50.0) * 4.0),
+              np.where((n // 40) % 2 == 0, np.sin(2 * np.pi * 0.05 * n), 0.0),
+              np.sign(np.sin(2 * np.pi * n / 30.0)),
+              rng.standard_normal(5), rng.standard_normal(2), np.zeros(0)]
+    T = [np.cumsum(rng.uniform(0.6, 1.4, len(x))) for x in traces]
+    ora = [O.emd(x, max_imf=15, max_extrema=1024, T=t) for x, t in zip(traces, T)]
+    times = [O.normalize_time(t) for t in T]
+    out = run_emu(emu, traces, 64, resident=2, persistent=True, times=times)
+    bad, _ = compare(list(traces), out, ora, 16, label="emu:time-axis-edge")
+    assert not [m for m in bad if not m[2]], bad
