@@ -454,13 +454,16 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
     plt |= sh.plat_w[q];
   }
   if (uni(plt)) {
-    // plateau present: one lane applies the full run rules and rewrites the masks
+    // plateau present: one lane applies the full run rules and rewrites the maxima
+    // masks, and one lane of another wave (the same lane when there is one wave) the
+    // minima masks; the two scans are independent
+    constexpr int kMinLane = NW > 1 ? 64 : 0;
     if (tid == 0) {
-      for (int b = 0; b < nblk; ++b) {
-        wk.mmax[b] = 0ull;
-        wk.mmin[b] = 0ull;
-      }
+      for (int b = 0; b < nblk; ++b) wk.mmax[b] = 0ull;
       serial_extrema(imf, N, true, [&](int idx) { wk.mmax[idx >> 6] |= uint64_t(1) << (idx & 63); });
+    }
+    if (tid == kMinLane) {
+      for (int b = 0; b < nblk; ++b) wk.mmin[b] = 0ull;
       serial_extrema(imf, N, false, [&](int idx) { wk.mmin[idx >> 6] |= uint64_t(1) << (idx & 63); });
     }
     __syncthreads();
