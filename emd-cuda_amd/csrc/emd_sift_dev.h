@@ -603,17 +603,25 @@ __device__ __forceinline__ KnotEnds ends_from_masks(const Work& wk, int nblk, in
   for (int b0 = 0; b0 < nblk; b0 += 64) {
     const int b = b0 + lane;
     const int pm = b < nblk ? wk.pmax[b] : INT_MAX, pn = b < nblk ? wk.pmin[b] : INT_MAX;
-    for (int L = 0; L < 4 * F; ++L) {
-      const int side = L >= 2 * F ? 1 : 0;
-      const int r = L - side * 2 * F, half = r >= F ? 1 : 0, k = r - half * F;
+    // target L: side, first or last F, k-th of those
+    auto target = [&](int L, int FF) {
+      const int side = L >= 2 * FF ? 1 : 0;
+      const int r = L - side * 2 * FF, half = r >= FF ? 1 : 0, k = r - half * FF;
       const int n = side ? nn : nm;
-      const int g = max(0, min(half ? n - F + k : k, n - 1));
+      const int g = max(0, min(half ? n - FF + k : k, n - 1));
       const uint64_t le = __ballot((side ? pn : pm) <= g);
       if (le != 0ull && lane == L) {
         myblk = b0 + 63 - __builtin_clzll(le);
         myg = g;
         myside = side;
       }
+    };
+    if (F == 3) {
+      // nbsym = 2, the default: the targets' sides and slots are constants
+#pragma unroll
+      for (int L = 0; L < 12; ++L) target(L, 3);
+    } else {
+      for (int L = 0; L < 4 * F; ++L) target(L, F);
     }
   }
   if (lane < 4 * F && (myside ? nn : nm) > 0) {
