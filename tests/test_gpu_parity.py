@@ -79,6 +79,17 @@ def test_short_lengths(N):
     _assert_parity(x, _gpu_padded(x), _oracle(x))
 
 
+@pytest.mark.parametrize("nbsym", [1, 3, 4])
+def test_other_nbsym_parity(nbsym):
+    """Mirrors of 1, 3 and 4 points per side: the general mirror path (nbsym = 2 runs a
+    constant-folded instance of it)."""
+    from cuda_emd import EMDKernelConfig, synth
+    x = np.concatenate([synth.chirp_mix(12, 1024, seed=4),
+                        synth.seismic_traces(4, 2001, seed=5)[:, :1024]])
+    o = [O.emd(s, max_imf=15, nbsym=nbsym) for s in x]
+    _assert_parity(x, _gpu_padded(x, EMDKernelConfig(nbsym=nbsym)), o)
+
+
 def test_overflow_small_max_extrema():
     from cuda_emd import EMDKernelConfig, synth
     x = synth.chirp_mix()[:16]

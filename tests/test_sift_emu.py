@@ -38,7 +38,8 @@ def emu():
 
 
 def run_emu(exe, traces, wg, tile=0, resident=2, persistent=True, order=None,
-            max_extrema=1024, max_imfs=16, max_iteration=1000, timeout=1800, times=None):
+            max_extrema=1024, max_imfs=16, max_iteration=1000, timeout=1800, times=None,
+            nbsym=2):
     """Decompose `traces` (list of 1-D float64) in the emulator; returns the parity dict.
     times: one normalised time axis per trace (the time-axis kernels), or None."""
     lengths = np.array([len(t) for t in traces], np.int64)
@@ -48,7 +49,7 @@ def run_emu(exe, traces, wg, tile=0, resident=2, persistent=True, order=None,
     with tempfile.TemporaryDirectory() as d:
         fi, fo = os.path.join(d, "in"), os.path.join(d, "out")
         with open(fi, "wb") as f:
-            np.array([MAGIC, B, wg, tile, resident, int(persistent), max_extrema, 2,
+            np.array([MAGIC, B, wg, tile, resident, int(persistent), max_extrema, nbsym,
                       max_iteration, K, order is not None, times is not None],
                      np.int32).tofile(f)
             np.array([values.size], np.int64).tofile(f)
@@ -215,4 +216,16 @@ def test_time_axis_edge_signals(emu):
     times = [O.normalize_time(t) for t in T]
     out = run_emu(emu, traces, 64, resident=2, persistent=True, times=times)
     bad, _ = compare(list(traces), out, ora, 16, label="emu:time-axis-edge")
+    assert not [m for m in bad if not m[2]], bad
+
+
+@pytest.mark.parametrize("nbsym", [1, 3, 4])
+def test_other_nbsym(emu, nbsym):
+    """Mirrors of 1, 3 and 4 points per side (the general mirror path; nbsym = 2 has its
+    own constant-folded instance), against the oracle."""
+    from cuda_emd import synth
+    traces = [np.ascontiguousarray(x) for x in synth.chirp_mix(3, 300, seed=17)]
+    out = run_emu(emu, traces, 64, resident=2, persistent=True, nbsym=nbsym)
+    ora = [O.emd(x, max_imf=15, max_extrema=1024, nbsym=nbsym) for x in traces]
+    bad, _ = compare(list(traces), out, ora, 16, label=f"emu:nbsym{nbsym}")
     assert not [m for m in bad if not m[2]], bad
