@@ -1384,10 +1384,20 @@ __device__ __forceinline__ SiftStats envelope_update(const Work& wk,
   double nmax = -INFINITY, nmin = INFINITY;
   const bool nak = kind0 == kNotAKnot && kind1 == kNotAKnot;
   for (int sb = w; sb < nsb; sb += NW) {
-    double a[3] = {0.0, 0.0, 0.0}, z[3] = {0.0, 0.0, 0.0};  // blocks 0+1 and 2+3
+    // a: the running pair sum, first of blocks 0 and 1, then of blocks 2 and 3; z: blocks
+    // 0+1 once the second pair starts.  The sums are of squares (never -0), so 0 + v == v
+    // and the association stays (v0 + v1) + (v2 + v3) for every K
+    double a[3] = {0.0, 0.0, 0.0}, z[3] = {0.0, 0.0, 0.0};
     for (int q = 0; q < K; ++q) {
       const int c = sb * K + q;
       if (c >= nblk2) break;
+      if (q == 2) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          z[k] = a[k];
+          a[k] = 0.0;
+        }
+      }
       double v[3];
       // the common case (both envelopes not-a-knot) gets a loop without the small kinds
       if (nak)
@@ -1397,12 +1407,7 @@ __device__ __forceinline__ SiftStats envelope_update(const Work& wk,
         envelope_block<WG, false>(wk, t, ax, sh, N, c, lane, s0, s1, kind0, kind1, v, nmax,
                                   nmin);
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        if (q == 0) a[k] = v[k];
-        else if (q == 1) a[k] += v[k];
-        else if (q == 2) z[k] = v[k];
-        else z[k] += v[k];
-      }
+      for (int k = 0; k < 3; ++k) a[k] += v[k];
     }
     double s_new = a[0] + z[0], s_diff = a[1] + z[1], s_std = a[2] + z[2];
     wave_tree_sum3_last(s_new, s_diff, s_std);
