@@ -416,6 +416,11 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
 
   // ---- pass A: per-block lane masks of maxima/minima, zero crossings, plateaus --------
   int nz = 0, plat = 0;
+  // lanes whose sample is in the trace (i < N) and has a successor (i <= N - 2): every
+  // lane of every block but the last (N > 64 (nblk - 1)), so the last block's masks are
+  // made once, off the loop's scalar path
+  const int last = nblk - 1;
+  const uint64_t in_last = lanes_below(N - last * 64), inner_last = lanes_below(N - 1 - last * 64);
   for (int blk = w; blk < nblk; blk += NW) {
     const int i = blk * 64 + lane;
     // all three loads stay inside the IMF array (the neighbours are clamped at its ends);
@@ -425,9 +430,9 @@ __device__ __forceinline__ DetectResult detect(const Work& wk, int N, SiftShared
     const double next = imf[min(i + 1, 64 * wk.nbcap - 1)];
     const double d2 = next - x;
     const double d1 = x - prev;
-    // lanes whose sample is in the trace (i < N), has a successor (i <= N - 2) and, for
-    // extrema, a predecessor (i >= 1)
-    const uint64_t in = lanes_below(N - blk * 64), inner = lanes_below(N - 1 - blk * 64);
+    // extrema also need a predecessor (i >= 1)
+    const uint64_t in = blk == last ? in_last : ~uint64_t(0);
+    const uint64_t inner = blk == last ? inner_last : ~uint64_t(0);
     const uint64_t inner1 = blk == 0 ? inner & ~uint64_t(1) : inner;
     const uint64_t ext = fmask<kFOlt>(d1 * d2) & inner1;
     const uint64_t bm = ext & fmask<kFOgt>(d1), bn = ext & fmask<kFOlt>(d1);
