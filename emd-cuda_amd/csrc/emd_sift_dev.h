@@ -1498,10 +1498,14 @@ __device__ __forceinline__ void build_envelopes(const TileT<typename AX::Pos>& t
   // the true extrema into the knot tile: waves 1.. while wave 0 mirrors (the slots are
   // disjoint); wave 0 itself when it is the only one
   if (NW == 1 || w > 0) {
+    // the compaction waves take longer than the mirror: raised issue priority (below the
+    // mirror's) lets them go ahead of other workgroups' waves on their SIMDs
+    if (NW > 1) __builtin_amdgcn_s_setprio(1);
     uint64_t bad = 0;
     compact_knots(wk, N, t, ax, D.nm, D.nn, P.nbsym, bad, NW == 1 ? 0 : w - 1,
                   NW == 1 ? 1 : NW - 1);
     if (lane == 0) sh.bad_w[w] = bad != 0ull ? 1 : 0;
+    if (NW > 1) __builtin_amdgcn_s_setprio(0);
   }
   __syncthreads();
   if (tid == 0) {
